@@ -1,0 +1,121 @@
+// Small device helpers: wave64/block reductions, phase reduction.
+#pragma once
+
+#include "gls_internal.hpp"
+
+namespace gls {
+
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T u = __shfl_xor(v, o, kWave);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T u = __shfl_xor(v, o, kWave);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Block-wide reductions for blockDim.x == NT (multiple of 64).  `scratch`
+// must hold NT/64 values; the result is returned to every thread.
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T* scratch) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) scratch[w] = v;
+    __syncthreads();
+    T r = 0;
+#pragma unroll
+    for (int i = 0; i < NT / kWave; ++i) r += scratch[i];
+    __syncthreads();
+    return r;
+}
+template <int NT, typename T>
+__device__ __forceinline__ T block_min(T v, T* scratch) {
+    v = wave_min(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) scratch[w] = v;
+    __syncthreads();
+    T r = scratch[0];
+#pragma unroll
+    for (int i = 1; i < NT / kWave; ++i) r = scratch[i] < r ? scratch[i] : r;
+    __syncthreads();
+    return r;
+}
+template <int NT, typename T>
+__device__ __forceinline__ T block_max(T v, T* scratch) {
+    v = wave_max(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) scratch[w] = v;
+    __syncthreads();
+    T r = scratch[0];
+#pragma unroll
+    for (int i = 1; i < NT / kWave; ++i) r = scratch[i] > r ? scratch[i] : r;
+    __syncthreads();
+    return r;
+}
+
+// exp(2 pi i r) for a phase r given in revolutions as FP64: the integer part
+// is removed in FP64 (r - rint(r) in [-1/2, 1/2]) and only the reduced phase
+// is evaluated in FP32.
+__device__ __forceinline__ float2 expi_rev(double r) {
+    const float rr = (float)(r - rint(r));
+    float s, c;
+    sincospif(2.0f * rr, &s, &c);
+    return make_float2(c, s);
+}
+
+// GLS power from the six sums, closed form b^T A^-1 b / den in FP64
+// (algebraically equal to lspow / lspow_flmean / lspow0 of
+// cuvarbase/kernels/lomb.cu:36-141; invalid -> -1 as lomb.cu:79-80).
+__device__ __forceinline__ double gls_power(double C, double S, double C2, double S2, double YC,
+                                            double YS, double YY, double ybar, int mode,
+                                            double l0, double l1) {
+    double CC, SS, CS, bc, bs, den;
+    if (mode == GLS_MODE_FLOATING_MEAN) {
+        CC = 0.5 * (1.0 + C2) - C * C + l0;
+        SS = 0.5 * (1.0 - C2) - S * S + l1;
+        CS = 0.5 * S2 - C * S;
+        bc = YC;
+        bs = YS;
+        den = YY;
+    } else if (mode == GLS_MODE_STANDARD) {
+        CC = 0.5 * (1.0 + C2) + l0;
+        SS = 0.5 * (1.0 - C2) + l1;
+        CS = 0.5 * S2;
+        bc = YC + ybar * C;
+        bs = YS + ybar * S;
+        den = YY + ybar * ybar;
+    } else if (mode == GLS_MODE_WINDOW) {
+        CC = 0.5 * (1.0 + C2);
+        SS = 0.5 * (1.0 - C2);
+        CS = 0.5 * S2;
+        bc = 2.0 * C;
+        bs = 2.0 * S;
+        den = 1.0;
+    } else {
+        return -1.0;
+    }
+    const double D = CC * SS - CS * CS;
+    const double P = (SS * bc * bc + CC * bs * bs - 2.0 * CS * bc * bs) / (den * D);
+    if (!isfinite(P) || P < 0.0) return -1.0;
+    return P;
+}
+
+}  // namespace gls

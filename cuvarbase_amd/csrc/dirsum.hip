@@ -1,0 +1,128 @@
+// Direct-summation GLS (use_fft=False).
+//
+// Replaces lomb_dirsum (cuvarbase/kernels/lomb.cu:144-165): six O(N) sums per
+// frequency, then the power.  Each thread owns RF consecutive frequencies of
+// one curve; observations are staged through LDS in chunks of 256.  Per
+// (observation, thread) the phase of the first frequency is reduced mod 1 in
+// FP64 and evaluated once in FP32; the next RF-1 frequencies follow by
+// rotation with exp(2 pi i df (t - tref)) (one complex multiply each).  Sums
+// are accumulated in FP32 within a chunk and in FP64 across chunks.
+#include "device_util.hpp"
+#include "lds_fft.hpp"
+
+namespace gls {
+
+namespace {
+
+constexpr int NT = 256;
+
+template <int RF, int OUT>
+__global__ __launch_bounds__(NT) void dirsum_kernel(const CurveGeom* __restrict__ geom,
+                                                    const CurveState* __restrict__ state,
+                                                    const int32_t* __restrict__ tile_curve,
+                                                    const int32_t* __restrict__ tile_first,
+                                                    PointArrays pa, PlanParams pp,
+                                                    void* __restrict__ out) {
+    __shared__ double sdt[NT];
+    __shared__ float sw[NT];
+    __shared__ float syw[NT];
+    __shared__ float2 srot[NT];
+    const int tid = threadIdx.x;
+    const int ci = tile_curve[blockIdx.x];
+    const CurveGeom& G = geom[ci];
+    const int N = G.n_obs;
+    const int k0 = tile_first[blockIdx.x] + tid * RF;
+    const double fk0 = G.f0 + (double)k0 * G.df;
+
+    double acc[6][RF];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+        for (int r = 0; r < RF; ++r) acc[q][r] = 0.0;
+
+    for (int base = 0; base < N; base += NT) {
+        const int j = base + tid;
+        if (j < N) {
+            const double dt = pa.dt[G.pts + j];
+            sdt[tid] = dt;
+            sw[tid] = pa.w[G.pts + j];
+            syw[tid] = pa.yw[G.pts + j];
+            srot[tid] = expi_rev(G.df * dt);
+        }
+        __syncthreads();
+        const int cnt = min(NT, N - base);
+        float a[6][RF];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int r = 0; r < RF; ++r) a[q][r] = 0.f;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const double dt = sdt[jj];
+            const float w = sw[jj];
+            const float yw = syw[jj];
+            const float2 rot = srot[jj];
+            float2 z = expi_rev(fk0 * dt);
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                const float c = z.x, s = z.y;
+                a[0][r] += w * c;
+                a[1][r] += w * s;
+                a[2][r] += w * (c * c - s * s);
+                a[3][r] += w * (2.f * c * s);
+                a[4][r] += yw * c;
+                a[5][r] += yw * s;
+                if (r + 1 < RF) z = cmul(z, rot);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int r = 0; r < RF; ++r) acc[q][r] += (double)a[q][r];
+        __syncthreads();
+    }
+
+    const CurveState st = state[ci];
+#pragma unroll
+    for (int r = 0; r < RF; ++r) {
+        const int k = k0 + r;
+        if (k < G.nf) {
+            const double P = gls_power(acc[0][r], acc[1][r], acc[2][r], acc[3][r], acc[4][r],
+                                          acc[5][r], st.yy, st.ybar, pp.mode, pp.reg_c,
+                                          pp.reg_s);
+            if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+            else ((double*)out)[G.out + k] = P;
+        }
+    }
+}
+
+template <int RF>
+hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
+                           const int32_t* tile_curve, const int32_t* tile_first, int ntiles,
+                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
+                           hipStream_t s) {
+    if (out_f64)
+        hipLaunchKernelGGL((dirsum_kernel<RF, 1>), dim3(ntiles), dim3(NT), 0, s, d_geom, state,
+                           tile_curve, tile_first, pa, pp, power);
+    else
+        hipLaunchKernelGGL((dirsum_kernel<RF, 0>), dim3(ntiles), dim3(NT), 0, s, d_geom, state,
+                           tile_curve, tile_first, pa, pp, power);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
+                         const int32_t* tile_curve, const int32_t* tile_first, int ntiles,
+                         int rf, PointArrays pa, const PlanParams& pp, void* power, int out_f64,
+                         hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    switch (rf) {
+        case 1: return dirsum_dispatch<1>(d_geom, state, tile_curve, tile_first, ntiles, pa, pp, power, out_f64, s);
+        case 2: return dirsum_dispatch<2>(d_geom, state, tile_curve, tile_first, ntiles, pa, pp, power, out_f64, s);
+        case 4: return dirsum_dispatch<4>(d_geom, state, tile_curve, tile_first, ntiles, pa, pp, power, out_f64, s);
+        case 8: return dirsum_dispatch<8>(d_geom, state, tile_curve, tile_first, ntiles, pa, pp, power, out_f64, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace gls

@@ -1,0 +1,123 @@
+// Gaussian gridding in the reference geometry, and per-curve top-k peaks.
+#include "device_util.hpp"
+
+namespace gls {
+
+namespace {
+
+constexpr int NT = 256;
+
+// One thread per point, 2m+1 taps with global float atomics: the debug output
+// of nfft_adjoint_async(just_return_gridded_data=True)
+// (cuvarbase/cunfft.py:229-268; taps of cuvarbase/kernels/cunfft.cu:140-161),
+// with the cell coordinate formed in FP64.
+__global__ __launch_bounds__(NT) void gaussian_grid_kernel(const double* __restrict__ t,
+                                                           const double* __restrict__ y,
+                                                           int n_obs, int ng, int m, double b,
+                                                           double tmin, double scale,
+                                                           float* __restrict__ grid) {
+    const int j = blockIdx.x * NT + threadIdx.x;
+    if (j >= n_obs) return;
+    const double x = (t[j] - tmin) * scale;
+    if (!isfinite(x)) return;
+    const double yc = (double)ng * (x - floor(x));
+    const double fl = floor(yc);
+    const int U = (int)fl - m;
+    const double fr = yc - fl;
+    const double yv = y[j];
+    const double norm = 1.0 / sqrt(M_PI * b);
+    for (int k = 0; k <= 2 * m; ++k) {
+        const double d = (double)(m - k) + fr;
+        int cell = U + k;
+        cell %= ng;
+        if (cell < 0) cell += ng;
+        atomicAdd(&grid[cell], (float)(yv * exp(-d * d / b) * norm));
+    }
+}
+
+// Top-k of one curve's powers (descending, ties -> lower index), skipping
+// masked indices and invalid (-1 / NaN) powers: k rounds of a block argmax.
+__global__ __launch_bounds__(NT) void topk_kernel(const CurveGeom* __restrict__ geom,
+                                                  const float* __restrict__ power,
+                                                  const uint8_t* __restrict__ mask, int K,
+                                                  float* __restrict__ top_power,
+                                                  int32_t* __restrict__ top_index) {
+    __shared__ float sv[NT / kWave];
+    __shared__ int si[NT / kWave];
+    __shared__ int chosen[64];
+    const int c = blockIdx.x;
+    const CurveGeom& G = geom[c];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    for (int round = 0; round < K; ++round) {
+        float best = -INFINITY;
+        int bi = 0x7fffffff;
+        for (int k = tid; k < G.nf; k += NT) {
+            if (mask && mask[G.out + k]) continue;
+            const float v = power[G.out + k];
+            if (!(v >= 0.f)) continue;  // -1 sentinel and NaN
+            bool taken = false;
+            for (int q = 0; q < round; ++q) taken |= (chosen[q] == k);
+            if (taken) continue;
+            if (v > best || (v == best && k < bi)) {
+                best = v;
+                bi = k;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(best, o, kWave);
+            const int oi = __shfl_xor(bi, o, kWave);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            sv[wv] = best;
+            si[wv] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float bv = sv[0];
+            int bx = si[0];
+            for (int q = 1; q < NT / kWave; ++q) {
+                if (sv[q] > bv || (sv[q] == bv && si[q] < bx)) {
+                    bv = sv[q];
+                    bx = si[q];
+                }
+            }
+            const bool ok = bx != 0x7fffffff;
+            chosen[round] = ok ? bx : -1;
+            top_power[(int64_t)c * K + round] = ok ? bv : -1.f;
+            top_index[(int64_t)c * K + round] = ok ? bx : -1;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gaussian_grid(const double* t, const double* y, int n_obs, int ng, int m,
+                                double b, double tmin, double tmax, double spp, float* grid,
+                                hipStream_t s) {
+    hipError_t e = hipMemsetAsync(grid, 0, sizeof(float) * (size_t)ng, s);
+    if (e != hipSuccess) return e;
+    if (n_obs <= 0) return hipSuccess;
+    const double scale = 1.0 / (spp * (tmax - tmin));
+    hipLaunchKernelGGL(gaussian_grid_kernel, dim3((n_obs + NT - 1) / NT), dim3(NT), 0, s, t, y,
+                       n_obs, ng, m, b, tmin, scale, grid);
+    return hipGetLastError();
+}
+
+hipError_t launch_topk(const CurveGeom* d_geom, int ncurves, const float* power,
+                       const uint8_t* mask, int k, float* top_power, int32_t* top_index,
+                       hipStream_t s) {
+    if (ncurves <= 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(topk_kernel, dim3(ncurves), dim3(NT), 0, s, d_geom, power, mask, k,
+                       top_power, top_index);
+    return hipGetLastError();
+}
+
+}  // namespace gls

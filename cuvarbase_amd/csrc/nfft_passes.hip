@@ -1,0 +1,269 @@
+// The two fused passes of the batched NFFT-GLS pipeline.
+//
+// Grid of n = n1*n2 cells, cell u = n2*u1 + u2 (u1 row, u2 column).  The
+// unnormalised + sign transform G[l] = sum_u g[u] exp(2 pi i l u / n) is done
+// four-step:
+//   pass 1  per column u2:  A[k1,u2] = exp(2 pi i u2 k1/n) sum_u1 g[n2 u1+u2] exp(2 pi i u1 k1/n1)
+//   pass 2  per row k1:     G[k1 + n1 k2] = sum_u2 A[k1,u2] exp(2 pi i u2 k2/n2)
+// Pass 1 builds its columns directly in LDS by Gaussian spreading (the work of
+// fast_gaussian_grid, cuvarbase/kernels/cunfft.cu:115-163) from the points the
+// prep kernel bucketed by column block, so the spread grid never touches HBM.
+// A is stored column-major (A[u2*n1 + k1]) so pass 1 writes whole columns.
+// Pass 2 reads rows of the three grids of a curve (SYW, SW(f), SW(2f) -- all
+// indexed by the same output k), applies the Gaussian deconvolution
+// (normalize, cuvarbase/kernels/cunfft.cu:243-245) and forms the GLS power
+// (lspow, cuvarbase/kernels/lomb.cu:126-141, in closed form) without ever
+// writing the transforms.
+#include "device_util.hpp"
+#include "lds_fft.hpp"
+
+namespace gls {
+
+namespace {
+
+constexpr int NT = 256;
+
+// ---------------------------------------------------------------------------
+// pass 1: spread + column FFT + inter-pass twiddle
+// ---------------------------------------------------------------------------
+template <int P1, int C1>
+__global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__ geom,
+                                                   const int32_t* __restrict__ curves,
+                                                   int ngrids, int p2, PointArrays pa,
+                                                   const int32_t* __restrict__ bkt,
+                                                   float2* __restrict__ grids, int m,
+                                                   Twiddles tw) {
+    constexpr int N1 = 1 << P1;
+    constexpr int BS = lds_stride(C1);
+    __shared__ float2 tile[N1 * BS];
+    const int tid = threadIdx.x;
+    const int n2 = 1 << p2;
+    const int nb = n2 / C1;
+    const int bid = blockIdx.x;
+    const int jb = bid % nb;
+    const int gi = (bid / nb) % ngrids;
+    const int ci = curves[bid / (nb * ngrids)];
+    const CurveGeom& G = geom[ci];
+    const int p = G.p;
+    const int n = 1 << p;
+
+    for (int i = tid; i < N1 * BS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+
+    // point set: buckets jb-nbk .. jb (cyclic), at most two contiguous ranges
+    const int g = (gi == 2) ? 2 : 1;
+    const int32_t* bk = bkt + G.bkt + (int64_t)(g - 1) * (nb + 1);
+    const int nbk = (2 * m + C1 - 1) / C1;
+    int s0, e0, s1, e1;
+    const int first = jb - nbk;
+    if (first >= 0) {
+        s0 = bk[first];
+        e0 = bk[jb + 1];
+        s1 = e1 = 0;
+    } else {
+        s0 = bk[nb + first];
+        e0 = bk[nb];
+        s1 = bk[0];
+        e1 = bk[jb + 1];
+    }
+    const int len0 = e0 - s0;
+    const int total = len0 + (e1 - s1);
+    const int32_t* Ua = (g == 1) ? pa.U1 : pa.U2;
+    const float* Fa = (g == 1) ? pa.fr1 : pa.fr2;
+    const float2* Va = (gi == 0) ? pa.v0 : (gi == 1 ? pa.v1 : pa.v2);
+    const float binv = (float)(1.4426950408889634 / G.b);   // log2(e)/b
+    const float norm = (float)(1.0 / sqrt(M_PI * G.b));
+    const int a = jb * C1;
+    const int two_m = 2 * m;
+    for (int q = tid; q < total; q += NT) {
+        const int idx = q < len0 ? s0 + q : s1 + (q - len0);
+        const int64_t o = G.pts + idx;
+        const int U = Ua[o];
+        const float fr = Fa[o];
+        const float2 v = Va[o];
+        const int col0 = U & (n2 - 1);
+#pragma unroll
+        for (int jj = 0; jj < C1; ++jj) {
+            const int k = (a + jj - col0) & (n2 - 1);
+            if (k <= two_m) {
+                int cell = U + k;
+                cell = cell < 0 ? cell + n : (cell >= n ? cell - n : cell);
+                const int u1 = cell >> p2;
+                const float d = (float)(m - k) + fr;
+                const float wgt = exp2f(-d * d * binv) * norm;
+                atomicAdd(&tile[u1 * BS + jj].x, v.x * wgt);
+                atomicAdd(&tile[u1 * BS + jj].y, v.y * wgt);
+            }
+        }
+    }
+    __syncthreads();
+
+    lds_fft<N1, C1, BS, NT>(tile, tw, tid);
+
+    float2* out = grids + G.grid + (int64_t)gi * n;
+    for (int i = tid; i < N1 * C1; i += NT) {
+        const int k1 = i & (N1 - 1);
+        const int jj = i >> P1;
+        const int u2 = a + jj;
+        const float2 w = twiddle_pow2(tw, (uint32_t)u2 * (uint32_t)k1, p);
+        out[(int64_t)u2 * N1 + k1] = cmul(tile[k1 * BS + jj], w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: row FFT + deconvolution + GLS assembly (or NFFT output)
+// ---------------------------------------------------------------------------
+// OUT: 0 = f32 power, 1 = f64 power, 2 = complex64 NFFT (single grid)
+template <int P2, int OUT>
+__global__ __launch_bounds__(NT) void pass2_kernel(const CurveGeom* __restrict__ geom,
+                                                   const CurveState* __restrict__ state,
+                                                   const int32_t* __restrict__ curves,
+                                                   int p1, const float2* __restrict__ grids,
+                                                   PlanParams pp, void* __restrict__ out,
+                                                   Twiddles tw) {
+    constexpr int N2 = 1 << P2;
+    constexpr int CR = pass2_rows(P2);
+    constexpr int NG = (OUT == 2) ? 1 : 3;
+    constexpr int B = NG * CR;
+    constexpr int BS = lds_stride(B);
+    __shared__ float2 tile[N2 * BS];
+    const int tid = threadIdx.x;
+    const int n1 = 1 << p1;
+    const int nrb = n1 / CR;
+    const int bid = blockIdx.x;
+    const int rb = bid % nrb;
+    const int ci = curves[bid / nrb];
+    const CurveGeom& G = geom[ci];
+    const int p = G.p;
+    const int n = 1 << p;
+    const int r0 = rb * CR;
+
+    const float2* src = grids + G.grid;
+    for (int i = tid; i < N2 * B; i += NT) {
+        const int bb = i % B;
+        const int u2 = i / B;
+        const int gi = bb / CR;
+        const int r = bb - gi * CR;
+        tile[u2 * BS + bb] = src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r];
+    }
+    __syncthreads();
+
+    lds_fft<N2, B, BS, NT>(tile, tw, tid);
+
+    const int K = G.nf;
+    const int kc = G.kc;
+    const float bpi2 = (float)(G.b * M_PI * M_PI);
+    const float ninv = 1.0f / (float)n;
+    const CurveState st = state[ci];
+    for (int i = tid; i < N2 * CR; i += NT) {
+        const int r = i % CR;
+        const int k2 = i / CR;
+        const int l = r0 + r + n1 * k2;
+        int lp;
+        if (l < K - kc) lp = l;
+        else if (l >= n - kc) lp = l - n;
+        else continue;
+        const float xi = (float)lp * ninv;
+        const float dc = expf(bpi2 * xi * xi);
+        const int k = lp + kc;
+        if (OUT == 2) {
+            const float2 V = cscale(tile[k2 * BS + r], dc);
+            // restore the absolute time origin: exp(2 pi i f_k tref)
+            const double fk = G.f0 + (double)k * G.df;
+            const float2 ph = expi_rev(fk * st.tref);
+            ((float2*)out)[G.out + k] = cmul(V, ph);
+        } else {
+            const float2 syw = tile[k2 * BS + r];
+            const float2 w1 = tile[k2 * BS + CR + r];
+            const float2 w2 = tile[k2 * BS + 2 * CR + r];
+            const double d = (double)dc;
+            const double P = gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x,
+                                       d * syw.y, st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+            if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+            else ((double*)out)[G.out + k] = P;
+        }
+    }
+}
+
+template <int P1>
+hipError_t pass1_dispatch(int p, const CurveGeom* d_geom, const int32_t* curves, int ncurves,
+                          int ngrids, PointArrays pa, const int32_t* bkt, float2* grids, int m,
+                          Twiddles tw, hipStream_t s) {
+    constexpr int C1 = pass1_cols(P1);
+    const int p2 = p - P1;
+    const int nb = (1 << p2) / C1;
+    const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pass1_kernel<P1, C1>), dim3((unsigned)nblocks), dim3(NT), 0, s, d_geom,
+                       curves, ngrids, p2, pa, bkt, grids, m, tw);
+    return hipGetLastError();
+}
+
+template <int P2, int OUT>
+hipError_t pass2_dispatch(int p1, const CurveGeom* d_geom, const CurveState* state,
+                          const int32_t* curves, int ncurves, const float2* grids,
+                          const PlanParams& pp, void* out, Twiddles tw, hipStream_t s) {
+    constexpr int CR = pass2_rows(P2);
+    const int64_t nblocks = (int64_t)ncurves * ((1 << p1) / CR);
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pass2_kernel<P2, OUT>), dim3((unsigned)nblocks), dim3(NT), 0, s, d_geom,
+                       state, curves, p1, grids, pp, out, tw);
+    return hipGetLastError();
+}
+
+template <int OUT>
+hipError_t pass2_switch(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+                        const int32_t* curves, int ncurves, const float2* grids,
+                        const PlanParams& pp, void* out, Twiddles tw, hipStream_t s) {
+    switch (p - p1) {
+        case 6: return pass2_dispatch<6, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 7: return pass2_dispatch<7, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 8: return pass2_dispatch<8, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 9: return pass2_dispatch<9, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 10: return pass2_dispatch<10, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 11: return pass2_dispatch<11, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        case 12: return pass2_dispatch<12, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pass1(int p, int p1, int c1, const CurveGeom* d_geom, const int32_t* curves,
+                        int ncurves, int ngrids, PointArrays pa, const int32_t* bkt,
+                        float2* grids, int m, Twiddles tw, hipStream_t s) {
+    if (ncurves <= 0) return hipSuccess;
+    if (c1 != pass1_cols(p1)) return hipErrorInvalidValue;
+    switch (p1) {
+        case 4: return pass1_dispatch<4>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 5: return pass1_dispatch<5>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 6: return pass1_dispatch<6>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 7: return pass1_dispatch<7>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 8: return pass1_dispatch<8>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 9: return pass1_dispatch<9>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 10: return pass1_dispatch<10>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 11: return pass1_dispatch<11>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        case 12: return pass1_dispatch<12>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pass2_lomb(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+                             const int32_t* curves, int ncurves, const float2* grids,
+                             const PlanParams& pp, void* power, int out_f64, Twiddles tw,
+                             hipStream_t s) {
+    if (ncurves <= 0) return hipSuccess;
+    if (out_f64)
+        return pass2_switch<1>(p, p1, d_geom, state, curves, ncurves, grids, pp, power, tw, s);
+    return pass2_switch<0>(p, p1, d_geom, state, curves, ncurves, grids, pp, power, tw, s);
+}
+
+hipError_t launch_pass2_nfft(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+                             const int32_t* curves, int ncurves, const float2* grids,
+                             float2* out, Twiddles tw, hipStream_t s) {
+    if (ncurves <= 0) return hipSuccess;
+    PlanParams pp{};
+    return pass2_switch<2>(p, p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+}
+
+}  // namespace gls

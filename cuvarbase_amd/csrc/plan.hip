@@ -1,0 +1,437 @@
+// Batch planning and execution (C ABI).
+//
+// A plan fixes, for every curve of a batch, the grid size of its three NFFTs
+// (n = 2^p >= sigma*nf, at least 2^10), the four-step split n = n1*n2, the
+// Gaussian width b and the workspace layout.  Curves with the same n form a
+// size bucket; each bucket runs as two launches (pass 1 and pass 2) after one
+// prep launch for the whole batch.  This replaces the reference's per-curve
+// allocation, two cuFFT plans per curve and nine launches per curve
+// (cuvarbase/lombscargle.py:129-181, :579-689; cuvarbase/cunfft.py:82-92,
+// :163-308).
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "gls_internal.hpp"
+
+struct gls_plan {
+    int kind = 0;  // 0 lomb, 1 nfft
+    gls::PlanParams pp{};
+    double sigma = 0.0;
+    int ncurves = 0;
+    std::vector<gls::CurveGeom> geom;
+    gls::CurveGeom* d_geom = nullptr;
+    struct Bucket {
+        int p = 0, p1 = 0, c1 = 0;
+        int64_t first = 0;  // offset into d_bucket_curves
+        int count = 0;
+    };
+    std::vector<Bucket> buckets;
+    int32_t* d_bucket_curves = nullptr;
+    int32_t* d_tiles = nullptr;  // direct method: tile_curve[ntiles], tile_first[ntiles]
+    int ntiles = 0;
+    int rf = 1;
+    size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
+           off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
+           off_grid = 0;
+    size_t ws_bytes = 0;
+    int64_t total_out = 0;
+};
+
+namespace gls {
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Layout {
+    size_t cur = 0;
+    size_t take(size_t bytes) {
+        const size_t o = cur;
+        cur = align_up(cur + bytes, 256);
+        return o;
+    }
+};
+
+int next_log2(double v) {
+    int p = 0;
+    while ((double)(1ull << p) < v && p < 62) ++p;
+    return p;
+}
+
+double gauss_b(double sigma, int m) {
+    // b = 2 sigma m / ((2 sigma - 1) pi)  (cuvarbase/cunfft.py:56-57)
+    return 2.0 * sigma * (double)m / ((2.0 * sigma - 1.0) * M_PI);
+}
+
+// common planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft)
+int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts) {
+    const int m = P->pp.m;
+    std::map<int, std::vector<int32_t>> by_p;
+    int64_t bkt_ints = 0, grid_elems = 0;
+    for (int i = 0; i < P->ncurves; ++i) {
+        CurveGeom& G = P->geom[i];
+        const int p = std::max(kMinLog2Grid, next_log2(std::ceil(P->sigma * (double)G.nf)));
+        if (p > kMaxLog2Grid)
+            return fail(GLS_E_UNSUPPORTED,
+                        "grid of curve " + std::to_string(i) + " exceeds 2^24 cells (nf=" +
+                            std::to_string(G.nf) + ")");
+        const int p1 = pass1_log2_len(p);
+        const int c1 = pass1_cols(p1);
+        const int n2 = 1 << (p - p1);
+        if (2 * m + c1 > n2)
+            return fail(GLS_E_UNSUPPORTED, "m too large for the grid of curve " + std::to_string(i));
+        const double n = (double)(1ll << p);
+        G.p = p;
+        G.p1 = p1;
+        G.c1 = c1;
+        G.kc = G.nf / 2;
+        G.b = gauss_b(n / (double)G.nf, m);
+        G.bkt = bkt_ints;
+        bkt_ints += 2 * (n2 / c1 + 1);
+        G.grid = grid_elems;
+        grid_elems += (int64_t)ngrids << p;
+        by_p[p].push_back(i);
+    }
+    P->off_U1 = L.take(sizeof(int32_t) * total_pts);
+    P->off_fr1 = L.take(sizeof(float) * total_pts);
+    P->off_v0 = L.take(sizeof(float2) * total_pts);
+    if (ngrids == 3) {
+        P->off_v1 = L.take(sizeof(float2) * total_pts);
+        P->off_U2 = L.take(sizeof(int32_t) * total_pts);
+        P->off_fr2 = L.take(sizeof(float) * total_pts);
+        P->off_v2 = L.take(sizeof(float2) * total_pts);
+    }
+    P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
+    P->off_grid = L.take(sizeof(float2) * grid_elems);
+    std::vector<int32_t> flat;
+    for (auto& kv : by_p) {
+        gls_plan::Bucket B;
+        B.p = kv.first;
+        B.p1 = pass1_log2_len(B.p);
+        B.c1 = pass1_cols(B.p1);
+        B.first = (int64_t)flat.size();
+        B.count = (int)kv.second.size();
+        flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+        P->buckets.push_back(B);
+    }
+    if (!flat.empty()) {
+        GLS_HIP_TRY(hipMalloc(&P->d_bucket_curves, sizeof(int32_t) * flat.size()));
+        GLS_HIP_TRY(hipMemcpy(P->d_bucket_curves, flat.data(), sizeof(int32_t) * flat.size(),
+                              hipMemcpyHostToDevice));
+    }
+    return GLS_OK;
+}
+
+int plan_direct_tiles(gls_plan* P) {
+    int64_t total_f = 0;
+    for (auto& G : P->geom) total_f += G.nf;
+    int rf = 8;
+    while (rf > 1) {
+        int64_t tiles = 0;
+        for (auto& G : P->geom) tiles += (G.nf + 256 * rf - 1) / (256 * rf);
+        if (tiles >= 512) break;
+        rf >>= 1;
+    }
+    std::vector<int32_t> tc, tf;
+    for (int i = 0; i < P->ncurves; ++i) {
+        const int nf = P->geom[i].nf;
+        for (int k = 0; k < nf; k += 256 * rf) {
+            tc.push_back(i);
+            tf.push_back(k);
+        }
+    }
+    if (tc.size() > 0x7fffffff) return fail(GLS_E_UNSUPPORTED, "too many direct-sum tiles");
+    P->rf = rf;
+    P->ntiles = (int)tc.size();
+    if (P->ntiles > 0) {
+        GLS_HIP_TRY(hipMalloc(&P->d_tiles, sizeof(int32_t) * 2 * tc.size()));
+        GLS_HIP_TRY(hipMemcpy(P->d_tiles, tc.data(), sizeof(int32_t) * tc.size(),
+                              hipMemcpyHostToDevice));
+        GLS_HIP_TRY(hipMemcpy(P->d_tiles + tc.size(), tf.data(), sizeof(int32_t) * tf.size(),
+                              hipMemcpyHostToDevice));
+    }
+    (void)total_f;
+    return GLS_OK;
+}
+
+int finish_plan(gls_plan* P, Layout& L) {
+    P->ws_bytes = std::max<size_t>(L.cur, 256);
+    if (!P->geom.empty()) {
+        GLS_HIP_TRY(hipMalloc(&P->d_geom, sizeof(CurveGeom) * P->geom.size()));
+        GLS_HIP_TRY(hipMemcpy(P->d_geom, P->geom.data(), sizeof(CurveGeom) * P->geom.size(),
+                              hipMemcpyHostToDevice));
+    }
+    return GLS_OK;
+}
+
+void destroy_plan(gls_plan* P) {
+    if (!P) return;
+    if (P->d_geom) hipFree(P->d_geom);
+    if (P->d_bucket_curves) hipFree(P->d_bucket_curves);
+    if (P->d_tiles) hipFree(P->d_tiles);
+    delete P;
+}
+
+PointArrays point_arrays(const gls_plan* P, void* ws) {
+    char* base = (char*)ws;
+    PointArrays pa{};
+    pa.U1 = (int32_t*)(base + P->off_U1);
+    pa.fr1 = (float*)(base + P->off_fr1);
+    pa.v0 = (float2*)(base + P->off_v0);
+    pa.v1 = (float2*)(base + P->off_v1);
+    pa.U2 = (int32_t*)(base + P->off_U2);
+    pa.fr2 = (float*)(base + P->off_fr2);
+    pa.v2 = (float2*)(base + P->off_v2);
+    pa.w = (float*)(base + P->off_w);
+    pa.yw = (float*)(base + P->off_yw);
+    pa.dt = (double*)(base + P->off_dt);
+    return pa;
+}
+
+int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, void* power,
+                 int out_f64, void* ws, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!P) return fail(GLS_E_INVALID, "plan is NULL");
+    if (P->kind != 0) return fail(GLS_E_INVALID, "plan is not a Lomb-Scargle plan");
+    if (P->ncurves == 0) return GLS_OK;
+    if (!t || !a || !b || !power || !ws) return fail(GLS_E_INVALID, "NULL device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    char* base = (char*)ws;
+    CurveState* state = (CurveState*)(base + P->off_state);
+    PointArrays pa = point_arrays(P, ws);
+    int32_t* bkt = (int32_t*)(base + P->off_bkt);
+    float2* grids = (float2*)(base + P->off_grid);
+    GLS_HIP_TRY(launch_prep(P->d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, 0, s));
+    const Twiddles tw = device_twiddles();
+    if (P->pp.method == GLS_METHOD_NFFT) {
+        for (const auto& B : P->buckets) {
+            const int32_t* curves = P->d_bucket_curves + B.first;
+            GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, P->d_geom, curves, B.count, 3, pa, bkt,
+                                     grids, P->pp.m, tw, s));
+            GLS_HIP_TRY(launch_pass2_lomb(B.p, B.p1, P->d_geom, state, curves, B.count, grids,
+                                          P->pp, power, out_f64, tw, s));
+        }
+    } else {
+        GLS_HIP_TRY(launch_dirsum(P->d_geom, state, P->d_tiles, P->d_tiles + P->ntiles,
+                                  P->ntiles, P->rf, pa, P->pp, power, out_f64, s));
+    }
+    return GLS_OK;
+}
+
+}  // namespace
+}  // namespace gls
+
+using namespace gls;
+
+extern "C" {
+
+int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurves,
+                         const gls_params* params) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan || !params) return fail(GLS_E_INVALID, "plan or params is NULL");
+    *plan = nullptr;
+    if (ncurves < 0 || (ncurves > 0 && !curves)) return fail(GLS_E_INVALID, "bad curve list");
+    const gls_params& prm = *params;
+    if (prm.mode < 0 || prm.mode > 2) return fail(GLS_E_INVALID, "mode must be 0, 1 or 2");
+    if (prm.input_kind != GLS_INPUT_RAW && prm.input_kind != GLS_INPUT_WEIGHTED)
+        return fail(GLS_E_INVALID, "unknown input_kind");
+    if (prm.method != GLS_METHOD_NFFT && prm.method != GLS_METHOD_DIRECT)
+        return fail(GLS_E_INVALID, "unknown method");
+    if (prm.method == GLS_METHOD_NFFT) {
+        if (!(prm.sigma >= 1.0 && prm.sigma <= 64.0))
+            return fail(GLS_E_INVALID, "sigma must be in [1, 64]");
+        if (prm.m < 1 || prm.m > 24) return fail(GLS_E_INVALID, "m must be in [1, 24]");
+    }
+    if (!std::isfinite(prm.reg_c) || !std::isfinite(prm.reg_s) || prm.reg_c < 0 || prm.reg_s < 0)
+        return fail(GLS_E_INVALID, "regularisation must be finite and >= 0");
+    gls_plan* P = new gls_plan();
+    P->kind = 0;
+    P->pp.m = prm.m;
+    P->pp.mode = prm.mode;
+    P->pp.reg_c = prm.reg_c;
+    P->pp.reg_s = prm.reg_s;
+    P->pp.input_kind = prm.input_kind;
+    P->pp.method = prm.method;
+    P->pp.kind = 0;
+    P->sigma = prm.sigma;
+    P->ncurves = ncurves;
+    P->geom.resize(ncurves);
+    int64_t pts = 0;
+    for (int i = 0; i < ncurves; ++i) {
+        const gls_curve& c = curves[i];
+        if (c.n_obs < 0 || c.nf < 1 || c.obs_offset < 0 || c.out_offset < 0 ||
+            !std::isfinite(c.f0) || !std::isfinite(c.df) || !(c.df > 0.0)) {
+            destroy_plan(P);
+            return fail(GLS_E_INVALID, "invalid curve descriptor " + std::to_string(i));
+        }
+        CurveGeom& G = P->geom[i];
+        G = CurveGeom{};
+        G.obs = c.obs_offset;
+        G.out = c.out_offset;
+        G.pts = pts;
+        G.f0 = c.f0;
+        G.df = c.df;
+        G.ybar_in = c.ybar;
+        G.yy_in = c.yy;
+        G.n_obs = c.n_obs;
+        G.nf = c.nf;
+        pts += c.n_obs;
+        P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
+    }
+    Layout L;
+    P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
+    if (prm.method == GLS_METHOD_NFFT) {
+        rc = plan_nfft_geometry(P, 3, L, pts);
+    } else {
+        P->off_w = L.take(sizeof(float) * pts);
+        P->off_yw = L.take(sizeof(float) * pts);
+        P->off_dt = L.take(sizeof(double) * pts);
+        rc = plan_direct_tiles(P);
+    }
+    if (rc == GLS_OK) rc = finish_plan(P, L);
+    if (rc != GLS_OK) {
+        destroy_plan(P);
+        return rc;
+    }
+    *plan = P;
+    return GLS_OK;
+}
+
+int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t nitems,
+                         double sigma, int32_t m) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    *plan = nullptr;
+    if (nitems < 0 || (nitems > 0 && !items)) return fail(GLS_E_INVALID, "bad item list");
+    if (!(sigma >= 1.0 && sigma <= 64.0)) return fail(GLS_E_INVALID, "sigma must be in [1, 64]");
+    if (m < 1 || m > 24) return fail(GLS_E_INVALID, "m must be in [1, 24]");
+    gls_plan* P = new gls_plan();
+    P->kind = 1;
+    P->pp.m = m;
+    P->pp.mode = GLS_MODE_FLOATING_MEAN;
+    P->pp.method = GLS_METHOD_NFFT;
+    P->pp.kind = 1;
+    P->sigma = sigma;
+    P->ncurves = nitems;
+    P->geom.resize(nitems);
+    int64_t pts = 0;
+    for (int i = 0; i < nitems; ++i) {
+        const gls_nfft_item& c = items[i];
+        if (c.n_obs < 0 || c.nf < 1 || c.obs_offset < 0 || c.out_offset < 0 ||
+            !std::isfinite(c.f0) || !std::isfinite(c.df) || !(c.df > 0.0)) {
+            destroy_plan(P);
+            return fail(GLS_E_INVALID, "invalid NFFT item " + std::to_string(i));
+        }
+        CurveGeom& G = P->geom[i];
+        G = CurveGeom{};
+        G.obs = c.obs_offset;
+        G.out = c.out_offset;
+        G.pts = pts;
+        G.f0 = c.f0;
+        G.df = c.df;
+        G.n_obs = c.n_obs;
+        G.nf = c.nf;
+        pts += c.n_obs;
+        P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
+    }
+    Layout L;
+    P->off_state = L.take(sizeof(CurveState) * (size_t)nitems);
+    rc = plan_nfft_geometry(P, 1, L, pts);
+    if (rc == GLS_OK) rc = finish_plan(P, L);
+    if (rc != GLS_OK) {
+        destroy_plan(P);
+        return rc;
+    }
+    *plan = P;
+    return GLS_OK;
+}
+
+int gls_plan_destroy(gls_plan* plan) {
+    destroy_plan(plan);
+    return GLS_OK;
+}
+
+int gls_plan_workspace_bytes(const gls_plan* plan, size_t* bytes) {
+    if (!plan || !bytes) return fail(GLS_E_INVALID, "NULL argument");
+    *bytes = plan->ws_bytes;
+    return GLS_OK;
+}
+
+int gls_plan_output_count(const gls_plan* plan, int64_t* count) {
+    if (!plan || !count) return fail(GLS_E_INVALID, "NULL argument");
+    *count = plan->total_out;
+    return GLS_OK;
+}
+
+int gls_plan_grid_size(const gls_plan* plan, int32_t i, int64_t* n) {
+    if (!plan || !n) return fail(GLS_E_INVALID, "NULL argument");
+    if (i < 0 || i >= plan->ncurves) return fail(GLS_E_INVALID, "curve index out of range");
+    *n = (plan->pp.method == GLS_METHOD_NFFT) ? (1ll << plan->geom[i].p) : 0;
+    return GLS_OK;
+}
+
+int gls_lomb_execute(gls_plan* plan, const double* t, const void* a, const void* b, float* power,
+                     void* workspace, void* stream) {
+    return execute_lomb(plan, t, a, b, power, 0, workspace, stream);
+}
+
+int gls_lomb_execute_f64(gls_plan* plan, const double* t, const void* a, const void* b,
+                         double* power, void* workspace, void* stream) {
+    return execute_lomb(plan, t, a, b, power, 1, workspace, stream);
+}
+
+int gls_nfft_execute(gls_plan* plan, const double* t, const void* y, int32_t y_is_f32,
+                     float* out, void* workspace, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (plan->kind != 1) return fail(GLS_E_INVALID, "plan is not an NFFT plan");
+    if (plan->ncurves == 0) return GLS_OK;
+    if (!t || !y || !out || !workspace) return fail(GLS_E_INVALID, "NULL device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    char* base = (char*)workspace;
+    CurveState* state = (CurveState*)(base + plan->off_state);
+    PointArrays pa = point_arrays(plan, workspace);
+    int32_t* bkt = (int32_t*)(base + plan->off_bkt);
+    float2* grids = (float2*)(base + plan->off_grid);
+    GLS_HIP_TRY(launch_prep(plan->d_geom, plan->ncurves, state, t, y, y, pa, bkt, plan->pp,
+                            y_is_f32 ? 1 : 0, s));
+    const Twiddles tw = device_twiddles();
+    for (const auto& B : plan->buckets) {
+        const int32_t* curves = plan->d_bucket_curves + B.first;
+        GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, plan->d_geom, curves, B.count, 1, pa, bkt,
+                                 grids, plan->pp.m, tw, s));
+        GLS_HIP_TRY(launch_pass2_nfft(B.p, B.p1, plan->d_geom, state, curves, B.count, grids,
+                                      (float2*)out, tw, s));
+    }
+    return GLS_OK;
+}
+
+int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs, int32_t ng, int32_t m,
+                      double b, double tmin, double tmax, double spp, float* grid,
+                      void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (ng < 1 || n_obs < 0 || m < 0 || !(b > 0) || !grid || (n_obs > 0 && (!t || !y)))
+        return fail(GLS_E_INVALID, "invalid gridding arguments");
+    if (!(tmax > tmin) || !(spp > 0)) return fail(GLS_E_INVALID, "need tmax > tmin and spp > 0");
+    GLS_HIP_TRY(launch_gaussian_grid(t, y, n_obs, ng, m, b, tmin, tmax, spp, grid,
+                                     (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_topk(const gls_plan* plan, const float* power, const uint8_t* mask, int32_t k,
+             float* top_power, int32_t* top_index, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan || !power || !top_power || !top_index) return fail(GLS_E_INVALID, "NULL argument");
+    if (k < 1 || k > 64) return fail(GLS_E_INVALID, "k must be in [1, 64]");
+    GLS_HIP_TRY(launch_topk(plan->d_geom, plan->ncurves, power, mask, k, top_power, top_index,
+                            (hipStream_t)stream));
+    return GLS_OK;
+}
+
+}  // extern "C"

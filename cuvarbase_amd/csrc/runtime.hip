@@ -1,0 +1,276 @@
+// Context, errors, streams, events and memory of libgls.so (C ABI).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "gls_internal.hpp"
+
+namespace gls {
+
+namespace {
+thread_local std::string g_err;
+std::mutex g_mu;
+bool g_init = false;
+int g_device = -1;
+float2* g_tw_hi = nullptr;
+float2* g_tw_lo = nullptr;
+}  // namespace
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? GLS_E_OOM : GLS_E_HIP;
+}
+
+bool initialised() { return g_init; }
+
+int require_init() {
+    if (!g_init) return fail(GLS_E_NOT_INIT, "gls_init() has not been called");
+    return GLS_OK;
+}
+
+Twiddles device_twiddles() { return Twiddles{g_tw_hi, g_tw_lo}; }
+
+}  // namespace gls
+
+using namespace gls;
+
+extern "C" {
+
+const char* gls_last_error(void) { return g_err.c_str(); }
+
+int gls_version(int* major, int* minor, int* patch) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    if (patch) *patch = 0;
+    return GLS_OK;
+}
+
+int gls_device_count(int* count) {
+    if (!count) return fail(GLS_E_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return GLS_OK;
+}
+
+int gls_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_init && device == g_device) return GLS_OK;
+    if (g_init && device != g_device)
+        return fail(GLS_E_INVALID, "libgls is already initialised on another device");
+    int n = 0;
+    GLS_HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(GLS_E_INVALID, "device index out of range");
+    GLS_HIP_TRY(hipSetDevice(device));
+    // resident twiddle tables, built in FP64 on the host
+    std::vector<float2> hi(4096), lo(4096);
+    for (int i = 0; i < 4096; ++i) {
+        const double a = 2.0 * M_PI * (double)i / 4096.0;
+        hi[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        const double b = 2.0 * M_PI * (double)i / 16777216.0;
+        lo[i] = make_float2((float)std::cos(b), (float)std::sin(b));
+    }
+    // exact values on the axes
+    hi[0] = make_float2(1.f, 0.f);
+    hi[1024] = make_float2(0.f, 1.f);
+    hi[2048] = make_float2(-1.f, 0.f);
+    hi[3072] = make_float2(0.f, -1.f);
+    GLS_HIP_TRY(hipMalloc(&g_tw_hi, 4096 * sizeof(float2)));
+    GLS_HIP_TRY(hipMalloc(&g_tw_lo, 4096 * sizeof(float2)));
+    GLS_HIP_TRY(hipMemcpy(g_tw_hi, hi.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    GLS_HIP_TRY(hipMemcpy(g_tw_lo, lo.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    g_device = device;
+    g_init = true;
+    return GLS_OK;
+}
+
+int gls_shutdown(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_init) return GLS_OK;
+    hipFree(g_tw_hi);
+    hipFree(g_tw_lo);
+    g_tw_hi = g_tw_lo = nullptr;
+    g_init = false;
+    g_device = -1;
+    return GLS_OK;
+}
+
+int gls_device_info(char* name, int len, int* num_cus, size_t* total_mem) {
+    int rc = require_init();
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    GLS_HIP_TRY(hipGetDeviceProperties(&prop, g_device));
+    if (name && len > 0) {
+        std::snprintf(name, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (total_mem) *total_mem = prop.totalGlobalMem;
+    return GLS_OK;
+}
+
+int gls_stream_create(void** stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!stream) return fail(GLS_E_INVALID, "stream is NULL");
+    hipStream_t s;
+    GLS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return GLS_OK;
+}
+
+int gls_stream_destroy(void* stream) {
+    if (!stream) return GLS_OK;
+    GLS_HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_stream_synchronize(void* stream) {
+    GLS_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_stream_query(void* stream, int* done) {
+    if (!done) return fail(GLS_E_INVALID, "done is NULL");
+    hipError_t e = hipStreamQuery((hipStream_t)stream);
+    if (e == hipSuccess) {
+        *done = 1;
+        return GLS_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return GLS_OK;
+    }
+    return hip_fail(e, "hipStreamQuery");
+}
+
+int gls_event_create(void** event) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!event) return fail(GLS_E_INVALID, "event is NULL");
+    hipEvent_t ev;
+    GLS_HIP_TRY(hipEventCreate(&ev));
+    *event = (void*)ev;
+    return GLS_OK;
+}
+
+int gls_event_destroy(void* event) {
+    if (!event) return GLS_OK;
+    GLS_HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return GLS_OK;
+}
+
+int gls_event_record(void* event, void* stream) {
+    GLS_HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_event_synchronize(void* event) {
+    GLS_HIP_TRY(hipEventSynchronize((hipEvent_t)event));
+    return GLS_OK;
+}
+
+int gls_event_query(void* event, int* done) {
+    if (!done) return fail(GLS_E_INVALID, "done is NULL");
+    hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipSuccess) {
+        *done = 1;
+        return GLS_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return GLS_OK;
+    }
+    return hip_fail(e, "hipEventQuery");
+}
+
+int gls_event_elapsed_ms(void* start, void* stop, float* ms) {
+    if (!ms) return fail(GLS_E_INVALID, "ms is NULL");
+    GLS_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return GLS_OK;
+}
+
+int gls_device_synchronize(void) {
+    GLS_HIP_TRY(hipDeviceSynchronize());
+    return GLS_OK;
+}
+
+int gls_malloc(void** ptr, size_t bytes) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!ptr) return fail(GLS_E_INVALID, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipMalloc(ptr, bytes));
+    return GLS_OK;
+}
+
+int gls_free(void* ptr) {
+    if (!ptr) return GLS_OK;
+    GLS_HIP_TRY(hipFree(ptr));
+    return GLS_OK;
+}
+
+int gls_host_alloc(void** ptr, size_t bytes) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!ptr) return fail(GLS_E_INVALID, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return GLS_OK;
+}
+
+int gls_host_free(void* ptr) {
+    if (!ptr) return GLS_OK;
+    GLS_HIP_TRY(hipHostFree(ptr));
+    return GLS_OK;
+}
+
+int gls_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_memset_async(void* dst, int value, size_t bytes, void* stream) {
+    if (bytes == 0) return GLS_OK;
+    GLS_HIP_TRY(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    int rc = require_init();
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    GLS_HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return GLS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+/*
+ * gls.h -- C ABI of libgls.so, the MI355X (gfx950) generalized Lomb-Scargle
+ * periodogram via the adjoint NFFT.
+ *
+ * Plain extern "C" functions, plain pointers and sizes.  Every call returns an
+ * int status (GLS_OK == 0, negative GLS_E_* on failure); gls_last_error()
+ * gives a thread-local description of the last failure.  No C++ exception
+ * crosses this boundary.  Device pointers are plain addresses of device
+ * memory (from gls_malloc or any HIP allocation on the same device); streams
+ * are hipStream_t values passed as void*.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * johnh2o2/cuvarbase 0.3.0).  The reference binds its kernels through PyCUDA
+ * `prepare`d functions and cuFFT through scikit-cuda; this library replaces
+ * that whole launch sequence per curve (precompute_psi -> fast_gaussian_grid
+ * -> nfft_shift -> cuFFT inverse -> normalize, twice, then lomb) by a planned,
+ * batched pipeline over many curves.
+ */
+#ifndef CUVARBASE_AMD_GLS_H
+#define CUVARBASE_AMD_GLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GLS_OK              0
+#define GLS_E_INVALID      -1   /* bad argument (size, pointer, mode ...)     */
+#define GLS_E_HIP          -2   /* HIP runtime error                          */
+#define GLS_E_OOM          -3   /* device or host allocation failed           */
+#define GLS_E_NOT_INIT     -4   /* gls_init() not called                      */
+#define GLS_E_UNSUPPORTED  -5   /* size or option outside what is built       */
+#define GLS_E_DATA         -6   /* input data violates a documented precondition */
+
+/* ---- periodogram modes (cuvarbase/kernels/lomb.cu:13-15) --------------- */
+#define GLS_MODE_STANDARD       0   /* lspow0: no offset term                */
+#define GLS_MODE_FLOATING_MEAN  1   /* lspow_flmean: GLS with offset (default) */
+#define GLS_MODE_WINDOW         2   /* spectral window                       */
+
+/* ---- input kinds -------------------------------------------------------- */
+/* RAW: t (f64), y (f64), dy (f64).  The device normalises and weights each
+ * curve as normalize_light_curves + LombScargleMemory.setdata do
+ * (cuvarbase/utils.py:113-146, cuvarbase/lombscargle.py:235-293).          */
+#define GLS_INPUT_RAW       0
+/* WEIGHTED: t (f64), yw (f32), w (f32) already prepared on the host, with
+ * ybar and yy given per curve -- the argument set of the reference's `lomb`
+ * and `lomb_dirsum` kernels (cuvarbase/kernels/lomb.cu:144-147, :189-197).  */
+#define GLS_INPUT_WEIGHTED  1
+
+/* ---- methods ------------------------------------------------------------ */
+#define GLS_METHOD_NFFT     0   /* use_fft=True  (cuvarbase/lombscargle.py:660-684) */
+#define GLS_METHOD_DIRECT   1   /* use_fft=False (cuvarbase/lombscargle.py:643-659) */
+
+/* One light curve of a batch.  Observations of curve i are
+ * [obs_offset, obs_offset + n_obs) of the input arrays and must be sorted by
+ * time (ascending); its nf powers go to power[out_offset .. out_offset+nf).
+ * The frequency grid is f_k = f0 + k*df, k = 0..nf-1, evaluated exactly
+ * (no rounding of f0 to a multiple of df). */
+typedef struct gls_curve {
+    int64_t obs_offset;
+    int32_t n_obs;
+    int32_t nf;
+    double  f0;
+    double  df;
+    int64_t out_offset;
+    double  ybar;      /* GLS_INPUT_WEIGHTED only: weighted mean of y       */
+    double  yy;        /* GLS_INPUT_WEIGHTED only: weighted variance of y   */
+} gls_curve;
+
+typedef struct gls_params {
+    double  sigma;       /* NFFT oversampling; grid n >= sigma*nf (cuvarbase/cunfft.py:352) */
+    int32_t m;           /* Gaussian half-width in grid cells (cuvarbase/cunfft.py:353)     */
+    int32_t mode;        /* GLS_MODE_*                                                      */
+    double  reg_c;       /* amplitude-prior lambda on the cosine amplitude, reg[0]          */
+    double  reg_s;       /* amplitude-prior lambda on the sine amplitude,   reg[1]          */
+                         /* (cuvarbase/lombscargle.py:103-118)                              */
+    int32_t input_kind;  /* GLS_INPUT_*                                                     */
+    int32_t method;      /* GLS_METHOD_*                                                    */
+} gls_params;
+
+/* One adjoint-NFFT item: sum_j y_j exp(2 pi i (f0 + k df) t_j), k < nf. */
+typedef struct gls_nfft_item {
+    int64_t obs_offset;
+    int32_t n_obs;
+    int32_t nf;
+    double  f0;
+    double  df;
+    int64_t out_offset;  /* offset in complex elements */
+} gls_nfft_item;
+
+typedef struct gls_plan gls_plan;   /* opaque */
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the import-time context creation `import pycuda.autoprimaryctx`
+ * (cuvarbase/__init__.py:2): selects `device` and builds the resident
+ * twiddle tables.  Idempotent per device. */
+int gls_init(int device);
+int gls_shutdown(void);
+const char* gls_last_error(void);
+int gls_device_count(int* count);
+/* Reports name (<= len bytes), number of CUs and total memory in bytes. */
+int gls_device_info(char* name, int len, int* num_cus, size_t* total_mem);
+int gls_version(int* major, int* minor, int* patch);
+
+/* ---- streams and events: cuda.Stream / synchronize (cuvarbase/core.py:27-40) */
+int gls_stream_create(void** stream);
+int gls_stream_destroy(void* stream);
+int gls_stream_synchronize(void* stream);
+/* *done = 1 if all work on the stream has completed, else 0. */
+int gls_stream_query(void* stream, int* done);
+int gls_event_create(void** event);
+int gls_event_destroy(void* event);
+int gls_event_record(void* event, void* stream);
+int gls_event_synchronize(void* event);
+int gls_event_query(void* event, int* done);
+int gls_event_elapsed_ms(void* start, void* stop, float* ms);
+int gls_device_synchronize(void);
+
+/* ---- memory: gpuarray.zeros / set_async / get_async / aligned_zeros
+ *      (cuvarbase/lombscargle.py:129-191, :295-308) ----------------------- */
+int gls_malloc(void** ptr, size_t bytes);
+int gls_free(void* ptr);
+int gls_host_alloc(void** ptr, size_t bytes);   /* page-locked host memory */
+int gls_host_free(void* ptr);
+int gls_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream);
+int gls_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream);
+int gls_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
+int gls_memset_async(void* dst, int value, size_t bytes, void* stream);
+int gls_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---- GLS periodogram ---------------------------------------------------- */
+/* Plans a batch: chooses each curve's grid size (n >= sigma*nf, FFT-friendly)
+ * and lays out the workspace.  Replaces LombScargleMemory.allocate +
+ * NFFTMemory.allocate_grid + the two cuFFT plans per curve
+ * (cuvarbase/lombscargle.py:129-181, cuvarbase/cunfft.py:82-92).
+ * `curves` is read during the call only. */
+int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves,
+                         int32_t ncurves, const gls_params* params);
+int gls_plan_destroy(gls_plan* plan);
+/* Device workspace the plan needs (caller allocates, e.g. gls_malloc). */
+int gls_plan_workspace_bytes(const gls_plan* plan, size_t* bytes);
+/* Total number of output elements (sum of nf) the plan writes. */
+int gls_plan_output_count(const gls_plan* plan, int64_t* count);
+/* Grid size chosen for curve i (0 for the direct method). */
+int gls_plan_grid_size(const gls_plan* plan, int32_t i, int64_t* n);
+
+/* Runs the batch on `stream`: replaces lomb_scargle_async for every curve
+ * (cuvarbase/lombscargle.py:579-689: nfft_adjoint_async twice + `lomb`, or
+ * `lomb_dirsum` for the direct method).
+ *   t      device f64 observation times
+ *   a, b   RAW: y (f64), dy (f64);  WEIGHTED: yw (f32), w (f32)
+ *   power  device f32 output, sum(nf) values; invalid powers are -1
+ *          (cuvarbase/kernels/lomb.cu:79-80)
+ *   workspace  device buffer of gls_plan_workspace_bytes() bytes
+ * Asynchronous; returns after enqueueing. */
+int gls_lomb_execute(gls_plan* plan, const double* t, const void* a,
+                     const void* b, float* power, void* workspace,
+                     void* stream);
+
+/* Same, into an f64 power array (use_double=True,
+ * cuvarbase/lombscargle.py:96-101). */
+int gls_lomb_execute_f64(gls_plan* plan, const double* t, const void* a,
+                         const void* b, double* power, void* workspace,
+                         void* stream);
+
+/* ---- adjoint NFFT (NFFTAsyncProcess.run, cuvarbase/cunfft.py:507-542) --- */
+/* y is f64 (or f32 with y_is_f32 = 1); output is complex64 (interleaved
+ * re, im) at out_offset*2 floats per item. */
+int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items,
+                         int32_t nitems, double sigma, int32_t m);
+int gls_nfft_execute(gls_plan* plan, const double* t, const void* y,
+                     int32_t y_is_f32, float* out, void* workspace,
+                     void* stream);
+
+/* Gaussian gridding only, in the reference's one-sided geometry: real grid of
+ * ng = int(sigma*nf) cells, x = (t - tmin)/(spp (tmax - tmin)); the
+ * `just_return_gridded_data=True` output of nfft_adjoint_async
+ * (cuvarbase/cunfft.py:229-268).  grid must hold ng floats and is
+ * overwritten.  n_obs points from t (f64) and y (f64). */
+int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs,
+                      int32_t ng, int32_t m, double b, double tmin,
+                      double tmax, double spp, float* grid, void* stream);
+
+/* ---- peak extraction (next row: best frequencies, lombscargle.py:1099-1107) */
+/* For each curve of a finished plan: the top-k powers and their indices,
+ * ignoring indices where mask[out_offset + k] != 0 (mask may be NULL).
+ * top_power: ncurves*k f32, top_index: ncurves*k i32 (descending power,
+ * ties by lower index; -1 where fewer than k valid values). */
+int gls_topk(const gls_plan* plan, const float* power,
+             const uint8_t* mask, int32_t k, float* top_power,
+             int32_t* top_index, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUVARBASE_AMD_GLS_H */
