@@ -36,6 +36,15 @@ struct gls_plan {
            off_grid = 0;
     size_t ws_bytes = 0;
     int64_t total_out = 0;
+    // optional per-stage timing (events recorded between launches)
+    struct Stage {
+        int kind;      // 0 prep, 1 pass 1, 2 pass 2, 3 direct sums
+        int log2n;     // grid size of the bucket (0 for prep / direct)
+        double bytes;  // algorithmic bytes moved by the stage
+    };
+    std::vector<Stage> stages;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
 };
 
 namespace gls {
@@ -154,7 +163,41 @@ int plan_direct_tiles(gls_plan* P) {
     return GLS_OK;
 }
 
+// Algorithmic bytes per stage: what each kernel must read and write at
+// minimum for its role in the fused pipeline.
+void build_stage_list(gls_plan* P) {
+    P->stages.clear();
+    const int ngr = (P->kind == 1) ? 1 : 3;
+    double prep = 0.0;
+    for (auto& G : P->geom) {
+        const double in_bytes = (P->pp.input_kind == GLS_INPUT_RAW || P->kind == 1) ? 24.0 : 16.0;
+        const double rec = (P->pp.method == GLS_METHOD_NFFT) ? (ngr == 3 ? 40.0 : 16.0) : 16.0;
+        prep += (double)G.n_obs * (in_bytes + rec);
+    }
+    P->stages.push_back({0, 0, prep});
+    const double out_b = 4.0;
+    if (P->pp.method == GLS_METHOD_NFFT) {
+        for (const auto& B : P->buckets) {
+            double b1 = 0.0, b2 = 0.0;
+            // curves of the bucket are contiguous in bucket order; recompute from geometry
+            for (auto& G : P->geom) {
+                if (G.p != B.p) continue;
+                const double n = (double)(1ll << G.p);
+                b1 += ngr * (16.0 * G.n_obs + 8.0 * n);
+                b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : out_b) * G.nf;
+            }
+            P->stages.push_back({1, B.p, b1});
+            P->stages.push_back({2, B.p, b2});
+        }
+    } else {
+        double b = 0.0;
+        for (auto& G : P->geom) b += 16.0 * G.n_obs + out_b * G.nf;
+        P->stages.push_back({3, 0, b});
+    }
+}
+
 int finish_plan(gls_plan* P, Layout& L) {
+    build_stage_list(P);
     P->ws_bytes = std::max<size_t>(L.cur, 256);
     if (!P->geom.empty()) {
         GLS_HIP_TRY(hipMalloc(&P->d_geom, sizeof(CurveGeom) * P->geom.size()));
@@ -169,6 +212,7 @@ void destroy_plan(gls_plan* P) {
     if (P->d_geom) hipFree(P->d_geom);
     if (P->d_bucket_curves) hipFree(P->d_bucket_curves);
     if (P->d_tiles) hipFree(P->d_tiles);
+    for (auto e : P->events) hipEventDestroy(e);
     delete P;
 }
 
@@ -202,19 +246,29 @@ int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, voi
     PointArrays pa = point_arrays(P, ws);
     int32_t* bkt = (int32_t*)(base + P->off_bkt);
     float2* grids = (float2*)(base + P->off_grid);
+    int ev = 0;
+    auto mark = [&]() -> hipError_t {
+        if (!P->timing) return hipSuccess;
+        return hipEventRecord(P->events[ev++], s);
+    };
+    GLS_HIP_TRY(mark());
     GLS_HIP_TRY(launch_prep(P->d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, 0, s));
+    GLS_HIP_TRY(mark());
     const Twiddles tw = device_twiddles();
     if (P->pp.method == GLS_METHOD_NFFT) {
         for (const auto& B : P->buckets) {
             const int32_t* curves = P->d_bucket_curves + B.first;
             GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, P->d_geom, curves, B.count, 3, pa, bkt,
                                      grids, P->pp.m, tw, s));
+            GLS_HIP_TRY(mark());
             GLS_HIP_TRY(launch_pass2_lomb(B.p, B.p1, P->d_geom, state, curves, B.count, grids,
                                           P->pp, power, out_f64, tw, s));
+            GLS_HIP_TRY(mark());
         }
     } else {
         GLS_HIP_TRY(launch_dirsum(P->d_geom, state, P->d_tiles, P->d_tiles + P->ntiles,
                                   P->ntiles, P->rf, pa, P->pp, power, out_f64, s));
+        GLS_HIP_TRY(mark());
     }
     return GLS_OK;
 }
@@ -397,15 +451,24 @@ int gls_nfft_execute(gls_plan* plan, const double* t, const void* y, int32_t y_i
     PointArrays pa = point_arrays(plan, workspace);
     int32_t* bkt = (int32_t*)(base + plan->off_bkt);
     float2* grids = (float2*)(base + plan->off_grid);
+    int ev = 0;
+    auto mark = [&]() -> hipError_t {
+        if (!plan->timing) return hipSuccess;
+        return hipEventRecord(plan->events[ev++], s);
+    };
+    GLS_HIP_TRY(mark());
     GLS_HIP_TRY(launch_prep(plan->d_geom, plan->ncurves, state, t, y, y, pa, bkt, plan->pp,
                             y_is_f32 ? 1 : 0, s));
+    GLS_HIP_TRY(mark());
     const Twiddles tw = device_twiddles();
     for (const auto& B : plan->buckets) {
         const int32_t* curves = plan->d_bucket_curves + B.first;
         GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, plan->d_geom, curves, B.count, 1, pa, bkt,
                                  grids, plan->pp.m, tw, s));
+        GLS_HIP_TRY(mark());
         GLS_HIP_TRY(launch_pass2_nfft(B.p, B.p1, plan->d_geom, state, curves, B.count, grids,
                                       (float2*)out, tw, s));
+        GLS_HIP_TRY(mark());
     }
     return GLS_OK;
 }
@@ -420,6 +483,39 @@ int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs, int32_t n
     if (!(tmax > tmin) || !(spp > 0)) return fail(GLS_E_INVALID, "need tmax > tmin and spp > 0");
     GLS_HIP_TRY(launch_gaussian_grid(t, y, n_obs, ng, m, b, tmin, tmax, spp, grid,
                                      (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_plan_enable_timing(gls_plan* plan, int32_t enable) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (enable && plan->events.empty()) {
+        plan->events.resize(plan->stages.size() + 1);
+        for (auto& e : plan->events) GLS_HIP_TRY(hipEventCreate(&e));
+    }
+    plan->timing = enable != 0;
+    return GLS_OK;
+}
+
+int gls_plan_stage_count(const gls_plan* plan, int32_t* count) {
+    if (!plan || !count) return fail(GLS_E_INVALID, "NULL argument");
+    *count = (int32_t)plan->stages.size();
+    return GLS_OK;
+}
+
+int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* log2n, float* ms,
+                         double* bytes, int32_t max) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    const int ns = (int)plan->stages.size();
+    for (int i = 0; i < ns && i < max; ++i) {
+        if (kind) kind[i] = plan->stages[i].kind;
+        if (log2n) log2n[i] = plan->stages[i].log2n;
+        if (bytes) bytes[i] = plan->stages[i].bytes;
+        if (ms) {
+            ms[i] = 0.f;
+            if (plan->timing && (size_t)i + 1 < plan->events.size())
+                GLS_HIP_TRY(hipEventElapsedTime(&ms[i], plan->events[i], plan->events[i + 1]));
+        }
+    }
     return GLS_OK;
 }
 

@@ -74,6 +74,24 @@ class Plan:
         _lib.call("gls_plan_grid_size", self.handle, C.c_int32(i), C.byref(v))
         return v.value
 
+    def enable_timing(self, enable=True):
+        _lib.call("gls_plan_enable_timing", self.handle, C.c_int32(1 if enable else 0))
+
+    STAGE_NAMES = {0: "prep", 1: "pass1", 2: "pass2", 3: "dirsum"}
+
+    def stage_times(self):
+        """[(stage, log2n, ms, algorithmic bytes)] of the last timed execute
+        (call after the stream has completed)."""
+        n = C.c_int32(0)
+        _lib.call("gls_plan_stage_count", self.handle, C.byref(n))
+        k = (C.c_int32 * max(n.value, 1))()
+        p = (C.c_int32 * max(n.value, 1))()
+        ms = (C.c_float * max(n.value, 1))()
+        by = (C.c_double * max(n.value, 1))()
+        _lib.call("gls_plan_stage_times", self.handle, k, p, ms, by, n)
+        return [(self.STAGE_NAMES.get(k[i], str(k[i])), p[i], ms[i], by[i])
+                for i in range(n.value)]
+
     def destroy(self):
         h, self.handle = getattr(self, "handle", None), None
         if h and _lib._lib is not None:

@@ -1,0 +1,602 @@
+"""Generalized Lomb-Scargle periodogram on MI355X (drop-in for
+cuvarbase/lombscargle.py).
+
+Public surface kept from the reference: ``get_k0``, ``check_k0``,
+``LombScargleMemory``, ``lomb_scargle_async``, ``LombScargleAsyncProcess``
+(``run``, ``allocate``, ``allocate_for_single_lc``, ``preallocate``,
+``batched_run``, ``batched_run_const_nfreq``, ``finish``, ``autofrequency``,
+``memory_requirement``), the FP64 host functions ``mhdirect_sums``,
+``add_regularization``, ``mhgls_params_from_sums``, ``mhgls_from_sums``,
+``lomb_scargle_direct_sums``, and ``fap_baluev``.
+
+Differences from the reference, by design:
+
+* The periodogram is evaluated at the exact frequencies ``f0 + k df`` of the
+  grid passed in; the reference evaluates the 2f sums at ``(2k + k0) df`` with
+  ``k0 = round(f0/df)`` (cuvarbase/kernels/lomb.cu:206).  ``check_k0`` is still
+  asserted so invalid grids fail the same way.
+* ``run(data)`` without preallocated memory plans all curves of ``data`` as
+  one batch (one launch sequence) instead of one stream per curve.
+* Times stay FP64 on the device; spreading/FFT are FP32, phases and the GLS
+  assembly FP64.  ``use_double=True`` returns float64 powers from the same
+  pipeline.
+* ``memory_requirement`` works (the reference raises UnboundLocalError,
+  cuvarbase/lombscargle.py:765) and the direct-sum path copies results back
+  under ``transfer_to_host`` (the reference tests ``transfer_to_device``,
+  cuvarbase/lombscargle.py:657).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+from scipy.special import gammaln
+
+from . import _lib
+from ._device import DeviceArray, pinned_zeros, synchronize, _handle
+from .core import GPUAsyncProcess
+from .cunfft import NFFTAsyncProcess, NFFTMemory
+from .engine import LombBatch, Plan
+from .utils import weights, normalize_light_curves
+from .utils import autofrequency as utils_autofreq
+
+
+def get_k0(freqs):
+    """max(1, round(f0/df))  (cuvarbase/lombscargle.py:25-26)."""
+    return max([1, int(round(freqs[0] / (freqs[1] - freqs[0])))])
+
+
+def check_k0(freqs, k0=None, rtol=1E-2, atol=1E-7):
+    """Assert |k0 df - f0| < rtol df + atol  (cuvarbase/lombscargle.py:29-33)."""
+    k0 = k0 if k0 is not None else get_k0(freqs)
+    df = freqs[1] - freqs[0]
+    f0 = k0 * df
+    assert abs(f0 - freqs[0]) < rtol * df + atol
+
+
+def _mode(floating_mean=True, window=False):
+    """mode = 1 floating mean / 0 standard / 2 window (cuvarbase/lombscargle.py:68-70)."""
+    if window:
+        return _lib.MODE_WINDOW
+    return _lib.MODE_FLOATING_MEAN if floating_mean else _lib.MODE_STANDARD
+
+
+def _lambda(amplitude_prior):
+    return 0.0 if amplitude_prior is None else float(np.power(amplitude_prior, -2.0))
+
+
+class LombScargleMemory(object):
+    """Device/host buffers of one light curve (cuvarbase/lombscargle.py:36-334).
+
+    Device arrays: ``t_g`` (f64), ``yw_g`` and ``w_g`` (f32), ``lsp_g``;
+    pinned host result ``lsp_c``.  ``nfft_mem_yw`` / ``nfft_mem_w`` are kept as
+    descriptors of the two transforms (their grids live in the plan
+    workspace of the fused pipeline, never as separate arrays).
+    """
+
+    def __init__(self, sigma, stream, m, **kwargs):
+        self.sigma = sigma
+        self.stream = stream
+        self.m = m
+        self.k0 = kwargs.get('k0', 0)
+        self.precomp_psi = kwargs.get('precomp_psi', True)
+        self.amplitude_prior = kwargs.get('amplitude_prior', None)
+        self.window = kwargs.get('window', False)
+        self.nharmonics = kwargs.get('nharmonics', 1)
+        self.use_fft = kwargs.get('use_fft', True)
+        self.other_settings = dict(kwargs)
+        self.floating_mean = kwargs.get('floating_mean', True)
+        self.use_double = kwargs.get('use_double', False)
+        self.mode = _mode(self.floating_mean, self.window)
+        self.n0 = kwargs.get('n0', None)
+        self.nf = kwargs.get('nf', None)
+        self.t_g = kwargs.get('t_g', None)
+        self.yw_g = kwargs.get('yw_g', None)
+        self.w_g = kwargs.get('w_g', None)
+        self.lsp_g = kwargs.get('lsp_g', None)
+        self.real_type = np.float64 if self.use_double else np.float32
+        self.complex_type = np.complex128 if self.use_double else np.complex64
+        if self.use_fft:
+            self.nfft_mem_yw = kwargs.get('nfft_mem_yw', None) or NFFTMemory(
+                self.sigma, self.stream, self.m, use_double=self.use_double)
+            self.nfft_mem_w = kwargs.get('nfft_mem_w', None) or NFFTMemory(
+                self.sigma, self.stream, self.m, use_double=self.use_double)
+        self.reg = np.zeros(2 * self.nharmonics + 1, dtype=self.real_type)
+        if self.amplitude_prior is not None:
+            lmbda = np.power(self.amplitude_prior, -2)
+            if isinstance(lmbda, float):
+                lmbda = lmbda * np.ones(self.nharmonics)
+            for i, l in enumerate(np.atleast_1d(lmbda)):
+                self.reg[2 * i] = self.real_type(l)
+                self.reg[1 + 2 * i] = self.real_type(l)
+        self.buffered_transfer = kwargs.get('buffered_transfer', False)
+        self.n0_buffer = kwargs.get('n0_buffer', None)
+        self.lsp_c = kwargs.get('lsp_c', None)
+        self.t = kwargs.get('t', None)
+        self.yw = kwargs.get('yw', None)
+        self.w = kwargs.get('w', None)
+        self.ybar = 0.
+        self.yy = 1.
+        self._plans = {}
+        self._ws = None
+
+    def _n0_alloc(self, **kwargs):
+        n0 = kwargs.get('n0', self.n0)
+        if self.buffered_transfer:
+            n0 = kwargs.get('n0_buffer', self.n0_buffer)
+        assert n0 is not None
+        return int(n0)
+
+    def allocate_data(self, **kwargs):
+        n0 = self._n0_alloc(**kwargs)
+        self.t_g = DeviceArray(max(n0, 1), np.float64, zero=True)
+        self.yw_g = DeviceArray(max(n0, 1), np.float32, zero=True)
+        self.w_g = DeviceArray(max(n0, 1), np.float32, zero=True)
+        if self.use_fft:
+            for mem, y in ((self.nfft_mem_yw, self.yw_g), (self.nfft_mem_w, self.w_g)):
+                mem.t_g, mem.y_g, mem.n0 = self.t_g, y, n0
+        return self
+
+    def allocate_grids(self, **kwargs):
+        k0 = kwargs.get('k0', self.k0)
+        self.nf = kwargs.get('nf', self.nf)
+        assert self.nf is not None
+        if self.use_fft:
+            fft_size = self.nharmonics * (self.nf + k0)
+            self.nfft_mem_yw.nf = fft_size - k0
+            self.nfft_mem_w.nf = 2 * fft_size - k0
+            self.nfft_mem_yw.n = int(self.sigma * self.nfft_mem_yw.nf)
+            self.nfft_mem_w.n = int(self.sigma * self.nfft_mem_w.nf)
+        self.lsp_g = DeviceArray(max(self.nf, 1), self.real_type, zero=True)
+        return self
+
+    def allocate_pinned_cpu(self, **kwargs):
+        nf = kwargs.get('nf', self.nf)
+        assert nf is not None
+        self.lsp_c = pinned_zeros(nf, self.real_type)
+        return self
+
+    def is_ready(self):
+        raise NotImplementedError()
+
+    def allocate_buffered_data_arrays(self, **kwargs):
+        n0 = self._n0_alloc(**kwargs)
+        self.t = np.zeros(n0, dtype=np.float64)
+        self.yw = np.zeros(n0, dtype=np.float32)
+        self.w = np.zeros(n0, dtype=np.float32)
+        return self
+
+    def allocate(self, **kwargs):
+        self.nf = kwargs.get('nf', self.nf)
+        assert self.nf is not None
+        self.allocate_data(**kwargs)
+        self.allocate_grids(**kwargs)
+        self.allocate_pinned_cpu(**kwargs)
+        if self.buffered_transfer:
+            self.allocate_buffered_data_arrays(**kwargs)
+        return self
+
+    def setdata(self, **kwargs):
+        """Weights, weighted mean and variance in FP64 on the host
+        (cuvarbase/lombscargle.py:235-293); yw and w are then stored as FP32,
+        t as FP64."""
+        t = kwargs.get('t', self.t)
+        yw = kwargs.get('yw', self.yw)
+        w = kwargs.get('w', self.w)
+        y = kwargs.get('y', None)
+        dy = kwargs.get('dy', None)
+        self.ybar = 0.
+        self.yy = kwargs.get('yy', 1.)
+        self.n0 = kwargs.get('n0', len(t))
+        if dy is not None:
+            assert 'w' not in kwargs
+            w = weights(dy)
+        if y is not None:
+            assert 'yw' not in kwargs
+            self.ybar = float(np.dot(y, w))
+            yw = np.multiply(w, y - self.ybar)
+            self.yy = float(np.dot(w, np.power(y - self.ybar, 2)))
+        t = np.asarray(t, dtype=np.float64)
+        yw = np.asarray(yw, dtype=np.float32)
+        w = np.asarray(w, dtype=np.float32)
+        if self.buffered_transfer:
+            if any(a is None for a in (self.t, self.yw, self.w)):
+                self.allocate_buffered_data_arrays(**kwargs)
+            assert self.n0 <= len(self.t)
+            self.t[:self.n0] = t[:self.n0]
+            self.yw[:self.n0] = yw[:self.n0]
+            self.w[:self.n0] = w[:self.n0]
+        else:
+            self.t, self.yw, self.w = t, yw, w
+        self.tmin = float(np.min(t[:self.n0])) if self.n0 else 0.0
+        self.tmax = float(np.max(t[:self.n0])) if self.n0 else 0.0
+        if self.use_fft:
+            for mem in (self.nfft_mem_yw, self.nfft_mem_w):
+                mem.tmin, mem.tmax, mem.n0 = self.tmin, self.tmax, self.n0
+        return self
+
+    def transfer_data_to_gpu(self, **kwargs):
+        assert not any(a is None for a in (self.t, self.yw, self.w))
+        n = self.n0
+        if n:
+            self.t_g.set_async(self.t[:n], stream=self.stream)
+            self.yw_g.set_async(self.yw[:n], stream=self.stream)
+            self.w_g.set_async(self.w[:n], stream=self.stream)
+
+    def transfer_lsp_to_cpu(self, **kwargs):
+        self.lsp_g.get_async(self.lsp_c, stream=self.stream, count=self.nf)
+
+    def fromdata(self, **kwargs):
+        self.setdata(**kwargs)
+        if kwargs.get('allocate', True):
+            self.allocate(**kwargs)
+        return self
+
+    def set_gpu_arrays_to_zero(self, **kwargs):
+        for x in (self.t_g, self.yw_g, self.w_g, self.lsp_g):
+            if x is not None:
+                x.fill(0, stream=self.stream)
+        for x in (self.t, self.yw, self.w):
+            if x is not None and self.buffered_transfer:
+                x[:] = 0.
+
+    def _plan(self, freqs, method):
+        f0, df, nf = float(freqs[0]), float(freqs[1] - freqs[0]), len(freqs)
+        key = (f0, df, nf, int(self.n0), method, self.mode, float(self.reg[0]),
+               float(self.reg[1]), float(self.ybar), float(self.yy))
+        p = self._plans.get(key)
+        if p is None:
+            c = _lib.gls_curve(obs_offset=0, n_obs=int(self.n0), nf=nf, f0=f0, df=df,
+                               out_offset=0, ybar=float(self.ybar), yy=float(self.yy))
+            p = Plan.lomb([c], self.sigma, self.m, self.mode, float(self.reg[0]),
+                          float(self.reg[1]), _lib.INPUT_WEIGHTED, method)
+            self._plans = {key: p}
+            need = p.workspace_bytes
+            if self._ws is None or self._ws.nbytes < need:
+                self._ws = DeviceArray(need, np.uint8)
+        return p
+
+
+class _LombLauncher(object):
+    """Callable bound in ``function_tuple`` for the reference's ``lomb`` /
+    ``lomb_dirsum`` kernels: runs one curve's planned pipeline on its stream."""
+
+    def __init__(self, name, method):
+        self.name = name
+        self.method = method
+
+    def __call__(self, memory, freqs):
+        plan = memory._plan(freqs, self.method)
+        fn = "gls_lomb_execute_f64" if memory.use_double else "gls_lomb_execute"
+        _lib.call(fn, plan.handle, memory.t_g.ptr, memory.yw_g.ptr, memory.w_g.ptr,
+                  memory.lsp_g.ptr, memory._ws.ptr, _handle(memory.stream))
+
+    def __repr__(self):
+        return f"<libgls {self.name}>"
+
+
+# ---------------------------------------------------------------------------
+# FP64 host sums (cuvarbase/lombscargle.py:337-576): the reference's CPU path,
+# reached only through python_dir_sums=True.
+# ---------------------------------------------------------------------------
+def mhdirect_sums(t, yw, w, freq, YY, nharms=1):
+    """C, S, CC, CS, SS, YC, YS of the H-harmonic model at one frequency,
+    phases reduced mod 1 in FP64 (cuvarbase/lombscargle.py:337-400)."""
+    phase = 2 * np.pi * ((np.asarray(t, np.float64) * freq) % 1.0)
+    hs = np.arange(1, nharms + 1)
+    ns = np.arange(2 * nharms + 1)
+    cosn = np.cos(np.multiply.outer(ns, phase))
+    sinn = np.sin(np.multiply.outer(ns, phase))
+    c = cosn @ w
+    s = sinn @ w
+    yc = cosn[hs] @ yw
+    ys = sinn[hs] @ yw
+    n, k = np.meshgrid(hs, hs, indexing='ij')
+    sgn = np.where(n - k == 0, 1.0, np.sign(n - k))
+    cc = 0.5 * (c[n + k] + c[np.abs(n - k)])
+    cs = 0.5 * (s[n + k] - sgn * s[np.abs(n - k)])
+    ss = 0.5 * (c[np.abs(n - k)] - c[n + k])
+    Cv, Sv = c[hs], s[hs]
+    ybar = np.sum(yw)
+    return (Cv, Sv, cc - np.outer(Cv, Cv), cs - np.outer(Cv, Sv), ss - np.outer(Sv, Sv),
+            yc - ybar * Cv, ys - ybar * Sv)
+
+
+def add_regularization(sums, amplitude_priors=None, cn0=None, sn0=None):
+    """Gaussian amplitude prior on the sums (cuvarbase/lombscargle.py:403-447)."""
+    C_, S, CC, CS, SS, YC, YS = sums
+    D = np.zeros_like(C_)
+    if amplitude_priors is not None:
+        D = np.ones_like(C_) * np.power(amplitude_priors, -2)
+    cn0 = np.zeros(len(C_)) if cn0 is None else cn0
+    sn0 = np.zeros(len(S)) if sn0 is None else sn0
+    return C_, S, CC + np.diag(D), CS, SS + np.diag(D), YC + D * cn0, YS + D * sn0
+
+
+def mhgls_params_from_sums(sums, YY, ybar):
+    """Amplitudes and offset (cuvarbase/lombscargle.py:450-494)."""
+    C_, S, CC, CS, SS, YC, YS = sums
+    H = len(C_)
+    A = np.block([[CC, CS], [CS.T, SS]])
+    theta = np.linalg.solve(A, np.concatenate((YC, YS)))
+    cn, sn = theta[:H], theta[H:]
+    return cn, sn, ybar - (np.dot(cn, C_) + np.dot(sn, S))
+
+
+def mhgls_from_sums(sums, YY, ybar):
+    """Periodogram power from the sums (cuvarbase/lombscargle.py:497-536)."""
+    C_, S, CC, CS, SS, YC, YS = sums
+    cn, sn, _ = mhgls_params_from_sums(sums, YY, ybar)
+    XX = np.outer(cn, cn) * CC + 2 * np.outer(cn, sn) * CS + np.outer(sn, sn) * SS
+    YX = 2 * (np.dot(cn, YC) + np.dot(sn, YS))
+    return (YX - np.sum(XX)) / YY
+
+
+def lomb_scargle_direct_sums(t, yw, w, freqs, YY, nharms=1, **kwargs):
+    """FP64 host periodogram by direct sums (cuvarbase/lombscargle.py:539-576)."""
+    sums = [add_regularization(mhdirect_sums(t, yw, w, f, YY, nharms=nharms), **kwargs)
+            for f in freqs]
+    ybar = np.sum(yw)
+    return np.array([mhgls_from_sums(s, YY, ybar) for s in sums])
+
+
+def lomb_scargle_async(memory, functions, freqs, block_size=256, use_fft=True,
+                       python_dir_sums=False, transfer_to_device=True,
+                       transfer_to_host=True, window=False, **kwargs):
+    """Asynchronous GLS periodogram of one curve (cuvarbase/lombscargle.py:579-689).
+
+    ``functions`` = ((lomb, lomb_dirsum), nfft_functions) as built by
+    LombScargleAsyncProcess; the result lands in ``memory.lsp_c`` and is
+    valid after the stream is synchronised.
+    """
+    (lomb, lomb_dirsum), nfft_funcs = functions
+    freqs = np.asarray(freqs, dtype=np.float64)
+    assert get_k0(freqs) == memory.k0
+    if transfer_to_device:
+        memory.transfer_data_to_gpu()
+    if python_dir_sums:
+        n = memory.n0
+        if memory.stream is not None:
+            memory.stream.synchronize()
+        t = memory.t_g.get(count=n)
+        yw = memory.yw_g.get(count=n).astype(np.float64)
+        w = memory.w_g.get(count=n).astype(np.float64)
+        return lomb_scargle_direct_sums(t, yw, w, freqs, memory.yy)
+    (lomb if use_fft else lomb_dirsum)(memory, freqs)
+    if transfer_to_host:
+        memory.transfer_lsp_to_cpu()
+    return memory.lsp_c
+
+
+class LombScargleAsyncProcess(GPUAsyncProcess):
+    """GPUAsyncProcess for the GLS periodogram (cuvarbase/lombscargle.py:692-1112).
+
+    ``**kwargs`` go to NFFTAsyncProcess (``sigma`` default 4, ``m`` 8,
+    ``autoset_m``, ``tol``, ``use_double``) and GPUAsyncProcess (``nstreams``,
+    ``device``).  ``nharmonics > 1`` raises, as in the reference.
+
+    Example
+    -------
+    >>> proc = LombScargleAsyncProcess()
+    >>> results = proc.run([(t, y, dy)])
+    >>> proc.finish()
+    >>> freqs, powers = results[0]
+    """
+
+    def __init__(self, *args, **kwargs):
+        super(LombScargleAsyncProcess, self).__init__(*args, **kwargs)
+        self.nfft_proc = NFFTAsyncProcess(*args, **kwargs)
+        self._cpp_defs = self.nfft_proc._cpp_defs
+        self.real_type = self.nfft_proc.real_type
+        self.complex_type = self.nfft_proc.complex_type
+        self.block_size = self.nfft_proc.block_size
+        self.module_options = self.nfft_proc.module_options
+        self.use_double = self.nfft_proc.use_double
+        self.memory = None
+        self.nharmonics = kwargs.get('nharmonics', 1)
+        if self.nharmonics > 1:
+            raise Exception("Only 1 harmonic is supported right now")
+
+    def _compile_and_prepare_functions(self, **kwargs):
+        self.nfft_proc._compile_and_prepare_functions(**kwargs)
+        self._select_device()
+        self.prepared_functions['lomb'] = _LombLauncher('lomb', _lib.METHOD_NFFT)
+        self.prepared_functions['lomb_dirsum'] = _LombLauncher('lomb_dirsum',
+                                                               _lib.METHOD_DIRECT)
+        self.function_tuple = tuple(self.prepared_functions[f]
+                                    for f in sorted(['lomb', 'lomb_dirsum']))
+
+    def _ready(self, **kwargs):
+        if not all(f in self.prepared_functions for f in ('lomb', 'lomb_dirsum')):
+            self._compile_and_prepare_functions(**kwargs)
+
+    def memory_requirement(self, n0, nf, k0, nbatch=1, autoadjust_sigma=False, **kwargs):
+        """Approximate device bytes for ``nbatch`` curves of ``n0`` points and
+        ``nf`` frequencies in this implementation."""
+        sigma = self.nfft_proc.sigma
+        rsize = np.dtype(self.real_type).itemsize
+        mem = (8 + 4 + 4) * n0 + rsize * nf
+        if kwargs.get('use_fft', True):
+            p = max(10, int(np.ceil(np.log2(max(sigma * nf, 1.0)))))
+            n = 1 << p
+            n2 = 1 << (p - min(12, max(4, p - 8)))
+            mem += 3 * n * 8 + 40 * n0 + 8 * (n2 + 1) + 64
+        else:
+            mem += 16 * n0 + 64
+        return int(mem * nbatch)
+
+    def allocate_for_single_lc(self, t, y, dy, nf, k0=0, stream=None, **kwargs):
+        m = self.nfft_proc.get_m(nf)
+        kw = dict(use_double=self.use_double, nharmonics=self.nharmonics)
+        kw.update(kwargs)
+        mem = LombScargleMemory(self.nfft_proc.sigma, stream, m, k0=k0, **kw)
+        mem.fromdata(t=t, y=y, dy=dy, nf=nf, allocate=True, **kwargs)
+        return mem
+
+    def preallocate(self, max_nobs, nlcs=1, nf=None, k0=None, freqs=None, streams=None,
+                    **kwargs):
+        if freqs is not None:
+            k0 = get_k0(freqs)
+            nf = len(freqs)
+        if nf is not None:
+            assert k0 is not None
+        m = self.nfft_proc.get_m(nf)
+        self.memory = []
+        for i in range(nlcs):
+            stream = None if streams is None else streams[i]
+            mem = LombScargleMemory(self.nfft_proc.sigma, stream, m, k0=k0,
+                                    buffered_transfer=True, n0_buffer=max_nobs, nf=nf,
+                                    use_double=self.use_double, nharmonics=self.nharmonics,
+                                    **kwargs)
+            mem.allocate(**kwargs)
+            self.memory.append(mem)
+        return self.memory
+
+    def autofrequency(self, *args, **kwargs):
+        return utils_autofreq(*args, **kwargs)
+
+    def _nfreqs(self, *args, **kwargs):
+        return len(self.autofrequency(*args, **kwargs))
+
+    def allocate(self, data, nfreqs=None, k0s=None, **kwargs):
+        if len(data) > len(self.streams):
+            self._create_streams(len(data) - len(self.streams))
+        nfrqs = nfreqs
+        if nfrqs is None:
+            nfrqs = [self._nfreqs(t, **kwargs) for (t, y, dy) in data]
+        elif isinstance(nfreqs, int):
+            nfrqs = [nfrqs] * len(data)
+        if k0s is None:
+            k0s = [1] * len(nfrqs)
+        elif isinstance(k0s, (int, float)):
+            k0s = [int(k0s)] * len(nfrqs)
+        return [self.allocate_for_single_lc(t, y, dy, int(nf), k0=k0, stream=self.streams[i],
+                                            **kwargs)
+                for i, ((t, y, dy), nf, k0) in enumerate(zip(data, nfrqs, k0s))]
+
+    def _frequencies(self, data, freqs, **kwargs):
+        frqs = freqs
+        if frqs is None:
+            frqs = [self.autofrequency(d[0], **kwargs) for d in data]
+        elif np.isscalar(frqs[0]):
+            frqs = [np.asarray(frqs)] * len(data)
+        assert len(frqs) == len(data)
+        k0s = [get_k0(frq) for frq in frqs]
+        for frq, k0 in zip(frqs, k0s):
+            check_k0(frq, k0=k0)
+        return frqs, k0s
+
+    def run(self, data, use_fft=True, memory=None, freqs=None, **kwargs):
+        """GLS periodograms of [(t, y, dy), ...]; returns [(freqs_i, P_i)]
+        with P_i valid after finish() (cuvarbase/lombscargle.py:923-1010)."""
+        self._ready(**kwargs)
+        data = normalize_light_curves(data)
+        frqs, k0s = self._frequencies(data, freqs, **kwargs)
+        if memory is None:
+            memory = self.memory
+        if memory is None:
+            return self._run_batch(data, frqs, use_fft=use_fft, **kwargs)
+        for i, (t, y, dy) in enumerate(data):
+            memory[i].set_gpu_arrays_to_zero(**kwargs)
+            memory[i].setdata(t=t, y=y, dy=dy, **kwargs)
+        ls_kwargs = dict(block_size=self.block_size, use_fft=use_fft)
+        ls_kwargs.update(kwargs)
+        funcs = (self.function_tuple, self.nfft_proc.function_tuple)
+        results = [lomb_scargle_async(memory[i], funcs, frqs[i], **ls_kwargs)
+                   for i in range(len(data))]
+        return [(f, r) for f, r in zip(frqs, results)]
+
+    def _run_batch(self, data, frqs, use_fft=True, python_dir_sums=False,
+                   transfer_to_host=True, **kwargs):
+        if python_dir_sums:
+            out = []
+            for (t, y, dy), f in zip(data, frqs):
+                w = weights(np.asarray(dy, np.float64))
+                ybar = np.dot(y, w)
+                yw = w * (y - ybar)
+                yy = np.dot(w, (y - ybar) ** 2)
+                out.append((f, lomb_scargle_direct_sums(t, yw, w, f, yy)))
+            return out
+        mode = _mode(kwargs.get('floating_mean', True), kwargs.get('window', False))
+        m = max(self.nfft_proc.get_m(len(f)) for f in frqs) if frqs else self.nfft_proc.m
+        batch = LombBatch(data, frqs, sigma=self.nfft_proc.sigma, m=m, mode=mode,
+                          amplitude_prior=kwargs.get('amplitude_prior', None),
+                          use_fft=use_fft, use_double=self.use_double, input_kind="raw")
+        if not self.streams:
+            self._create_streams(1)
+        s = self.streams[0]
+        batch.upload(stream=s)
+        batch.execute(stream=s)
+        flat = batch.download_async(stream=s) if transfer_to_host else \
+            pinned_zeros(max(batch.total_out, 1), batch.out_dtype)
+        self._inflight.append(batch)
+        return [(f, p) for f, p in zip(frqs, batch.split(flat))]
+
+    def batched_run_const_nfreq(self, data, batch_size=10, use_fft=True, freqs=None,
+                                only_return_best_freqs=False, ignore_freq_mask=None,
+                                **kwargs):
+        """Shared-grid batched run (cuvarbase/lombscargle.py:1012-1112)."""
+        self._ready(**kwargs)
+        data = normalize_light_curves(data)
+        if freqs is None:
+            longest = max(data, key=lambda d: max(d[0]) - min(d[0]))
+            freqs = self.autofrequency(longest[0], **kwargs)
+            df = freqs[1] - freqs[0]
+            k0 = get_k0(freqs)
+            nf = int(round(max(freqs) / df)) - k0
+            freqs = df * (k0 + np.arange(nf))
+        freqs = np.asarray(freqs, dtype=np.float64)
+        check_k0(freqs, k0=get_k0(freqs))
+        mask = np.ones(len(freqs), dtype=bool) if ignore_freq_mask is None else \
+            ~np.asarray(ignore_freq_mask)
+        lsps, best_freqs, best_sig = [], [], []
+        for start in range(0, len(data), batch_size):
+            batch = data[start:start + batch_size]
+            results = self.run(batch, freqs=freqs, use_fft=use_fft, **kwargs)
+            self.finish()
+            for i, (f, p) in enumerate(results):
+                if only_return_best_freqs:
+                    best_index = int(np.argmax(p[mask]))
+                    fap = fap_baluev(batch[i][0], batch[i][2], p[mask], np.max(freqs[mask]))
+                    best_freqs.append(freqs[mask][best_index])
+                    best_sig.append(1. - fap[best_index])
+                else:
+                    lsps.append(np.copy(p))
+        if only_return_best_freqs:
+            return best_freqs, best_sig
+        return [(freqs, lsp) for lsp in lsps]
+
+
+def fap_baluev(t, dy, z, fmax, d_K=3, d_H=1, use_gamma=True):
+    """Baluev (2008) false-alarm probability of periodogram values ``z``
+    (cuvarbase/lombscargle.py:1115-1184), including the reference's form of
+    the first exponent factor, (z/pi)^0.5 * (d-1) (:1177)."""
+    t = np.asarray(t, dtype=np.float64)
+    N = len(t)
+    d = d_K - d_H
+    N_K = N - d_K
+    N_H = N - d_H
+    g = (0.5 * N_H) ** (0.5 * (d - 1))
+    if use_gamma:
+        g = np.exp(gammaln(0.5 * N_H) - gammaln(0.5 * (N_K + 1)))
+    w = np.power(np.asarray(dy, dtype=np.float64), -2)
+    tbar = np.dot(w, t) / np.sum(w)
+    Dt = np.dot(w, np.power(t - tbar, 2)) / np.sum(w)
+    Teff = np.sqrt(4 * np.pi * Dt)
+    A = (2 * np.pi ** 1.5) * fmax * Teff
+    z = np.asarray(z, dtype=np.float64)
+    eZ1 = (z / np.pi) ** 0.5 * (d - 1)
+    eZ2 = (1 - z) ** (0.5 * (N_K - 1))
+    tau = (g * A / (2 * np.pi)) * eZ1 * eZ2
+    Psing = 1 - (1 - z) ** (0.5 * N_K)
+    return 1 - Psing * np.exp(-tau)
+
+
+def lomb_scargle_simple(t, y, dy, **kwargs):
+    """One-curve convenience call (cuvarbase/lombscargle.py:1187-1204), passing
+    ``dy`` where the reference passed normalised weights."""
+    proc = LombScargleAsyncProcess()
+    results = proc.run([(t, y, dy)], **kwargs)
+    proc.finish()
+    freqs, powers = results[0]
+    return freqs, np.array(powers)

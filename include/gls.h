@@ -55,8 +55,9 @@ extern "C" {
 #define GLS_METHOD_DIRECT   1   /* use_fft=False (cuvarbase/lombscargle.py:643-659) */
 
 /* One light curve of a batch.  Observations of curve i are
- * [obs_offset, obs_offset + n_obs) of the input arrays and must be sorted by
- * time (ascending); its nf powers go to power[out_offset .. out_offset+nf).
+ * [obs_offset, obs_offset + n_obs) of the input arrays (any order); its nf
+ * powers go to power[out_offset .. out_offset+nf).  Output ranges of
+ * different curves must not overlap.
  * The frequency grid is f_k = f0 + k*df, k = 0..nf-1, evaluated exactly
  * (no rounding of f0 to a multiple of df). */
 typedef struct gls_curve {
@@ -165,6 +166,19 @@ int gls_lomb_execute(gls_plan* plan, const double* t, const void* a,
 int gls_lomb_execute_f64(gls_plan* plan, const double* t, const void* a,
                          const void* b, double* power, void* workspace,
                          void* stream);
+
+/* ---- per-stage timing (no counterpart in the reference, whose only timing
+ *      is wall clock around whole calls, docs/source/plots/benchmarks.py:53-63).
+ * When enabled, gls_lomb_execute records HIP events on its stream between
+ * the launches of the plan's stages: prep, then per grid-size bucket pass 1
+ * and pass 2 (or the direct-sum launch).  After the stream has completed,
+ * gls_plan_stage_times reports for each stage its kind (0 prep, 1 pass 1,
+ * 2 pass 2, 3 direct sums), log2 grid size, elapsed ms and the algorithmic
+ * bytes it must move. */
+int gls_plan_enable_timing(gls_plan* plan, int32_t enable);
+int gls_plan_stage_count(const gls_plan* plan, int32_t* count);
+int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* log2n,
+                         float* ms, double* bytes, int32_t max);
 
 /* ---- adjoint NFFT (NFFTAsyncProcess.run, cuvarbase/cunfft.py:507-542) --- */
 /* y is f64 (or f32 with y_is_f32 = 1); output is complex64 (interleaved
