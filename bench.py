@@ -26,8 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-SIGMA_BENCH = 2.0
-M_BENCH = 8
+TOL = 1e-4
 
 
 def _cpu_worker(args):
@@ -83,6 +82,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=300)
     ap.add_argument("--parity-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sigma", type=float, default=None,
+                    help="NFFT oversampling (default: tuned for |dP| <= 1e-4)")
+    ap.add_argument("--m", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the GLS path "
@@ -90,8 +92,11 @@ def main():
               "GPU's share.", file=sys.stderr)
         sys.exit(2)
 
-    from cuvarbase_amd import synthetic
+    from cuvarbase_amd import synthetic, tuning
 
+    sigma, m = tuning.choose_nfft_params(TOL)
+    sigma = args.sigma if args.sigma is not None else sigma
+    m = args.m if args.m is not None else m
     data, freqs = synthetic.survey(args.ncurves)
     total_evals = int(sum(len(f) for f in freqs))
 
@@ -100,7 +105,7 @@ def main():
     from cuvarbase_amd import engine, _device
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
 
-    batch = engine.LombBatch(data, freqs, sigma=SIGMA_BENCH, m=M_BENCH)
+    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m)
     batch.plan.enable_timing(True)
     s = _device.Stream()
     batch.upload(stream=s)
@@ -138,20 +143,28 @@ def main():
         acc += ms
         if name == "pass2":
             ends[p] = acc
-    lat = []
-    for i in range(batch.ncurves):
-        lat.append(ends.get(int(math.log2(batch.plan.grid_size(i))), acc))
+    geo = [batch.plan.geometry(i) for i in range(batch.ncurves)]
+    lat = [ends.get(g[0], acc) for g in geo]
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
-    # dominant kernel: the pass with the most time (same template across buckets)
-    by_kind = {}
+    # dominant kernel: the kernel template (pass-1 column length n1 or pass-2
+    # row length n2) with the most time over the timed steps
+    split = {g[0]: (g[1], g[2]) for g in geo}
+    by_kernel, by_kind = {}, {}
     for (name, p), (c, ms, by) in stage_acc.items():
-        k = by_kind.setdefault(name, [0, 0.0, 0.0])
-        k[0] += c
-        k[1] += ms
-        k[2] += by
-    dom = max(by_kind, key=lambda k: by_kind[k][1])
-    c, ms, by = by_kind[dom]
+        if name == "pass1":
+            key = f"pass1_kernel<{split[p][0]}>"
+        elif name == "pass2":
+            key = f"pass2_kernel<{int(math.log2(split[p][1]))}, 0>"
+        else:
+            key = name
+        for d, k in ((by_kernel, key), (by_kind, name)):
+            a = d.setdefault(k, [0, 0.0, 0.0])
+            a[0] += c
+            a[1] += ms
+            a[2] += by
+    dom = max(by_kernel, key=lambda k: by_kernel[k][1])
+    c, ms, by = by_kernel[dom]
     achieved = (by / c) / (ms / c / 1e3) / 1e9
 
     # FP64 parity on a sample (independent high-accuracy oracle NFFT)
@@ -169,7 +182,7 @@ def main():
         same += int(np.argmax(P[i]) == np.argmax(Po))
 
     # plain path: the drop-in API, one curve per run() + finish()
-    proc = LombScargleAsyncProcess(sigma=SIGMA_BENCH, m=M_BENCH)
+    proc = LombScargleAsyncProcess(sigma=sigma, m=m)
     nplain = min(args.plain_sample, len(data))
     proc.run([data[0]], freqs=[freqs[0]])
     proc.finish()
@@ -179,12 +192,12 @@ def main():
         proc.run([data[i]], freqs=[freqs[i]])
         proc.finish()
         ev += len(freqs[i])
-    plain = ev / (time.perf_counter() - t0)
+    plain = ev / (time.perf_counter() - t0) if nplain else None
 
     # pipeline-level roofline: fixed algorithmic bytes B = 16N + 24G + 28nf
     nobs = np.array([len(d[0]) for d in data], dtype=np.float64)
     nfa = np.array([len(f) for f in freqs], dtype=np.float64)
-    G = np.ceil(SIGMA_BENCH * nfa) + np.ceil(SIGMA_BENCH * (2 * nfa + 1))
+    G = np.ceil(2.0 * nfa) + np.ceil(2.0 * (2 * nfa + 1))
     pipe_bytes = float(np.sum(16 * nobs + 24 * G + 28 * nfa))
 
     out = {
@@ -205,12 +218,15 @@ def main():
                         "U[30,3650] d, seasonal gaps, df=1/(5T), fmax=24/d",
             "curves": batch.ncurves,
             "evals_per_step": total_evals,
-            "sigma": SIGMA_BENCH,
-            "m": M_BENCH,
+            "sigma": sigma,
+            "m": m,
+            "grid_cells_per_step": int(3 * sum(g[0] for g in geo)),
         },
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
+            "launches_per_step": c // args.steps,
+            "avg_launch_ms": ms / c,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -223,8 +239,10 @@ def main():
             "frac": pipe_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
         "stages_ms_per_step": {k: v[1] / args.steps for k, v in by_kind.items()},
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in
+                                sorted(by_kernel.items(), key=lambda kv: -kv[1][1])[:8]},
         "plain_evals_per_s": plain,
-        "batched_vs_plain": value / plain,
+        "batched_vs_plain": (value / plain) if plain else None,
         "latency_ms": {"p50": p50, "p99": p99},
         "parity": {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(idx)}",
                    "oracle": "FP64 NumPy NFFT sigma=4 m=12"},

@@ -98,6 +98,8 @@ _PROTOS = {
     "gls_plan_workspace_bytes": [_vp, C.POINTER(C.c_size_t)],
     "gls_plan_output_count": [_vp, C.POINTER(C.c_int64)],
     "gls_plan_grid_size": [_vp, C.c_int32, C.POINTER(C.c_int64)],
+    "gls_plan_curve_geometry": [_vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "gls_lomb_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_nfft_plan_create": [C.POINTER(_vp), C.POINTER(gls_nfft_item), C.c_int32,

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/gls.h"
+#include "sizes.hpp"
 
 namespace gls {
 
@@ -27,21 +28,8 @@ int hip_fail(hipError_t e, const char* what);
 bool initialised();
 int require_init();
 
-// ---------------------------------------------------------------------------
-// resident twiddle tables (built by gls_init, one set per device)
-//   hi[i] = exp(+2 pi i * i / 4096),      i < 4096
-//   lo[i] = exp(+2 pi i * i / 2^24),      i < 4096
-// exp(2 pi i q / 2^p), q < 2^p <= 2^24:  Q = q << (24-p),
-//   = hi[Q >> 12] * lo[Q & 4095]
-// ---------------------------------------------------------------------------
-struct Twiddles {
-    const float2* hi;
-    const float2* lo;
-};
-Twiddles device_twiddles();
-
-constexpr int kMaxLog2Grid = 24;   // largest grid n = 2^24 cells
-constexpr int kMinLog2Grid = 10;   // smallest grid n = 2^10 cells
+constexpr int kMaxGrid = 1 << 24;   // largest grid n (cells)
+constexpr int kMinGrid = 1 << 10;   // smallest grid n
 
 // ---------------------------------------------------------------------------
 // per-curve plan data (host computed, uploaded once per plan)
@@ -51,7 +39,7 @@ struct CurveGeom {
     int64_t out;       // first output element
     int64_t pts;       // first point slot in the per-point workspace arrays
     int64_t bkt;       // first int of this curve's bucket table (2*(nb+1))
-    int64_t grid;      // first float2 of this curve's 3 grids (3*n)
+    int64_t grid;      // first float2 of this curve's grids (ngrids * n)
     double f0;
     double df;
     double b;          // Gaussian width in cells^2
@@ -59,10 +47,12 @@ struct CurveGeom {
     double yy_in;      // GLS_INPUT_WEIGHTED
     int32_t n_obs;
     int32_t nf;        // K, number of output frequencies
-    int32_t p;         // log2 of grid size n
-    int32_t p1;        // log2 of pass-1 column length n1
+    int32_t n;         // grid size n = n1 * n2
+    int32_t n1;        // pass-1 column length
+    int32_t p2;        // log2 of pass-2 row length n2
     int32_t c1;        // pass-1 columns per tile
     int32_t kc;        // K/2, band centre index
+    int32_t pad;
 };
 
 // written by the prep kernel
@@ -75,9 +65,9 @@ struct CurveState {
 
 // per-point workspace (structure of arrays, indexed by CurveGeom::pts + j)
 struct PointArrays {
-    int32_t* U1;   float* fr1;     // grid type 1 (SYW, W1): sorted by bucket
+    int32_t* U1;   float* fr1;     // node set 1 (SYW, W1): sorted by bucket
     float2* v0;    float2* v1;     // modulated yw, w
-    int32_t* U2;   float* fr2;     // grid type 2 (W2 at 2f)
+    int32_t* U2;   float* fr2;     // node set 2 (W2 at 2f)
     float2* v2;                    // modulated w at 2f
     float* w;      float* yw;      // prepared weights (direct method)
     double* dt;                    // t - tref (direct method)
@@ -93,24 +83,37 @@ struct PlanParams {
     int32_t kind;      // 0 = lomb, 1 = nfft
 };
 
+// pass-1 columns per tile: power of two with n1 * c1 <= 4096, at most 16
+constexpr int pass1_cols(int n1) {
+    int c = 1;
+    while (c < 16 && n1 * c * 2 <= 4096) c *= 2;
+    return c;
+}
+// pass-2 rows per tile: power of two with n2 * cr = 2048 where possible
+constexpr int pass2_rows(int p2) {
+    int c = 2048 >> p2;
+    if (c > 16) c = 16;
+    if (c < 1) c = 1;
+    return c;
+}
+
 // launchers (defined in the kernel translation units)
 hipError_t launch_prep(const CurveGeom* d_geom, int ncurves, CurveState* state,
                        const double* t, const void* a, const void* b,
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
                        int y_is_f32, hipStream_t s);
-hipError_t launch_pass1(int p, int p1, int c1, const CurveGeom* d_geom,
-                        const int32_t* curves, int ncurves, int ngrids,
-                        PointArrays pa, const int32_t* bkt, float2* grids,
-                        int m, Twiddles tw, hipStream_t s);
-hipError_t launch_pass2_lomb(int p, int p1, const CurveGeom* d_geom,
+hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves,
+                        int ncurves, int ngrids, int p2, PointArrays pa,
+                        const int32_t* bkt, float2* grids, int m, hipStream_t s);
+hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom,
                              const CurveState* state, const int32_t* curves,
                              int ncurves, const float2* grids,
                              const PlanParams& pp, void* power, int out_f64,
-                             Twiddles tw, hipStream_t s);
-hipError_t launch_pass2_nfft(int p, int p1, const CurveGeom* d_geom,
+                             hipStream_t s);
+hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom,
                              const CurveState* state, const int32_t* curves,
                              int ncurves, const float2* grids, float2* out,
-                             Twiddles tw, hipStream_t s);
+                             hipStream_t s);
 hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const int32_t* tile_curve, const int32_t* tile_first,
                          int ntiles, int rf, PointArrays pa,
@@ -123,27 +126,5 @@ hipError_t launch_gaussian_grid(const double* t, const double* y, int n_obs,
 hipError_t launch_topk(const CurveGeom* d_geom, int ncurves, const float* power,
                        const uint8_t* mask, int k, float* top_power,
                        int32_t* top_index, hipStream_t s);
-
-// size rules shared by planner and kernels
-inline int pass1_log2_len(int p) {
-    // n = n1 * n2, n1 = 2^p1 column length (pass 1), n2 = 2^(p-p1) row length
-    int p1 = p - 8;
-    if (p1 > 12) p1 = 12;
-    if (p1 < 4) p1 = 4;
-    if (p - p1 < 6) p1 = p - 6;
-    return p1;
-}
-constexpr int pass1_cols(int p1) {
-    int c = 8192 >> p1;
-    if (c > 16) c = 16;
-    if (c < 1) c = 1;
-    return c;
-}
-constexpr int pass2_rows(int p2) {
-    int c = 2048 >> p2;
-    if (c > 16) c = 16;
-    if (c < 1) c = 1;
-    return c;
-}
 
 }  // namespace gls

@@ -1,9 +1,12 @@
-// LDS-resident Stockham FFT building blocks for gfx950 (wave64).
+// LDS-resident mixed-radix Stockham FFT building blocks for gfx950 (wave64).
 //
 // All transforms use the + sign of an unnormalised inverse DFT,
 //   X[k] = sum_u x[u] exp(+2 pi i u k / L),
 // the convention the reference obtains from cuFFT's inverse C2C
 // (cuvarbase/cunfft.py:288, pinned by cuvarbase/tests/test_nfft.py:186-200).
+// Lengths are 2^a 3^b 5^c <= 4096; stages use radices 16, 8, 4, 2, 5, 3.
+// Stage twiddles come from a two-level table of exp(2 pi i x / L) in LDS
+// (64 + ceil(L/64) entries) and integer powers by complex products.
 #pragma once
 
 #include "gls_internal.hpp"
@@ -16,10 +19,21 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
 
 constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v >> 1); }
 constexpr int brev_c(int v, int bits) {
     return bits == 0 ? 0 : ((v & 1) << (bits - 1)) | brev_c(v >> 1, bits - 1);
+}
+constexpr int next_radix(int rem) {
+    return rem % 16 == 0 ? 16 : rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2
+         : rem % 5 == 0 ? 5 : rem % 3 == 0 ? 3 : rem;
+}
+constexpr bool smooth235(int v) {
+    while (v % 2 == 0) v /= 2;
+    while (v % 3 == 0) v /= 3;
+    while (v % 5 == 0) v /= 5;
+    return v == 1;
 }
 
 // v * exp(+2 pi i e / 16) with e known after unrolling (constants fold)
@@ -47,46 +61,102 @@ __device__ __forceinline__ float2 mul_w16(float2 v, int e) {
     }
 }
 
-// In-register DFT of size R in {2,4,8,16}: x[r] <- sum_q x[q] exp(+2 pi i q r / R)
+// In-register DFT: x[r] <- sum_q x[q] exp(+2 pi i q r / R)
 template <int R>
 __device__ __forceinline__ void dft_reg(float2 (&x)[R]) {
-    constexpr int LOG = ilog2_c(R);
-    float2 y[R];
+    if constexpr (R == 3) {
+        constexpr float h = 0.86602540378443864676f;   // sqrt(3)/2
+        const float2 t1 = cadd(x[1], x[2]);
+        const float2 t2 = csub(x[1], x[2]);
+        const float2 m1 = make_float2(x[0].x - 0.5f * t1.x, x[0].y - 0.5f * t1.y);
+        const float2 m2 = make_float2(-h * t2.y, h * t2.x);
+        x[0] = cadd(x[0], t1);
+        x[1] = cadd(m1, m2);
+        x[2] = csub(m1, m2);
+    } else if constexpr (R == 5) {
+        constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+        constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+        constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+        constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+        const float2 t1 = cadd(x[1], x[4]);
+        const float2 t2 = cadd(x[2], x[3]);
+        const float2 t3 = csub(x[1], x[4]);
+        const float2 t4 = csub(x[2], x[3]);
+        const float2 a1 = make_float2(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+        const float2 a2 = make_float2(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+        const float2 b1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+        const float2 b2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+        x[0] = cadd(x[0], cadd(t1, t2));
+        x[1] = cadd(a1, b1);
+        x[4] = csub(a1, b1);
+        x[2] = cadd(a2, b2);
+        x[3] = csub(a2, b2);
+    } else {
+        constexpr int LOG = ilog2_c(R);
+        float2 y[R];
 #pragma unroll
-    for (int i = 0; i < R; ++i) y[brev_c(i, LOG)] = x[i];
+        for (int i = 0; i < R; ++i) y[brev_c(i, LOG)] = x[i];
 #pragma unroll
-    for (int len = 2; len <= R; len <<= 1) {
-        const int half = len >> 1;
+        for (int len = 2; len <= R; len <<= 1) {
+            const int half = len >> 1;
 #pragma unroll
-        for (int i = 0; i < R; i += len) {
+            for (int i = 0; i < R; i += len) {
 #pragma unroll
-            for (int j = 0; j < half; ++j) {
-                float2 u = y[i + j];
-                float2 v = mul_w16(y[i + j + half], j * (16 / len));
-                y[i + j] = cadd(u, v);
-                y[i + j + half] = csub(u, v);
+                for (int j = 0; j < half; ++j) {
+                    float2 u = y[i + j];
+                    float2 v = mul_w16(y[i + j + half], j * (16 / len));
+                    y[i + j] = cadd(u, v);
+                    y[i + j + half] = csub(u, v);
+                }
             }
         }
-    }
 #pragma unroll
-    for (int i = 0; i < R; ++i) x[i] = y[i];
+        for (int i = 0; i < R; ++i) x[i] = y[i];
+    }
 }
 
-// exp(2 pi i q / 2^p), q < 2^p, p <= 24, from the resident two-level table
-__device__ __forceinline__ float2 twiddle_pow2(const Twiddles& tw, uint32_t q, int p) {
-    uint32_t Q = q << (24 - p);
-    float2 h = tw.hi[Q >> 12];
-    uint32_t lo = Q & 4095u;
-    return lo == 0 ? h : cmul(h, tw.lo[lo]);
+// exp(2 pi i q / n) for an integer q (any sign) and 0 < n <= 2^24, evaluated
+// from the exactly reduced ratio (|2q'| <= n is exact in FP32).
+__device__ __forceinline__ float2 expi_ratio(int64_t q, int64_t n) {
+    int64_t r = q % n;
+    if (r < 0) r += n;
+    if (2 * r >= n) r -= n;
+    float s, c;
+    sincospif((float)(2 * r) / (float)n, &s, &c);
+    return make_float2(c, s);
+}
+
+// two-level table of exp(2 pi i x / L), x < L: hi[x >> 6] * lo[x & 63]
+template <int L>
+struct TwTable {
+    static constexpr int NH = (L + 63) / 64;
+    float2 hi[NH];
+    float2 lo[64];
+};
+
+template <int L, int NT>
+__device__ __forceinline__ void build_twiddles(TwTable<L>& T, int tid) {
+    for (int i = tid; i < 64 + TwTable<L>::NH; i += NT) {
+        if (i < 64) T.lo[i] = expi_ratio(i, L);
+        else T.hi[i - 64] = expi_ratio((int64_t)(i - 64) * 64, L);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ float2 tw_lookup(const TwTable<L>& T, int x) {
+    if (L <= 64) return T.lo[x];
+    const float2 h = T.hi[x >> 6];
+    const int l = x & 63;
+    return l == 0 ? h : cmul(h, T.lo[l]);
 }
 
 // One radix-R Stockham stage over B interleaved sequences of length L held in
 // LDS as s[i*BS + b].  NS = product of the radices of earlier stages.
 template <int L, int B, int BS, int R, int NS, int NT>
-__device__ __forceinline__ void stockham_stage(float2* s, const Twiddles& tw, int tid) {
-    constexpr int NBUT = (L / R) * B;                 // butterflies in the stage
+__device__ __forceinline__ void stockham_stage(float2* s, const TwTable<L>& T, int tid) {
+    constexpr int LR = L / R;
+    constexpr int NBUT = LR * B;                      // butterflies in the stage
     constexpr int NB = (NBUT + NT - 1) / NT;          // per thread
-    constexpr int LOGL = ilog2_c(L);
     float2 x[NB][R];
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
@@ -95,15 +165,19 @@ __device__ __forceinline__ void stockham_stage(float2* s, const Twiddles& tw, in
             const int b = beta % B;
             const int j = beta / B;
 #pragma unroll
-            for (int q = 0; q < R; ++q) x[r][q] = s[(j + q * (L / R)) * BS + b];
-            if (NS > 1) {
-                const int k = j & (NS - 1);
+            for (int q = 0; q < R; ++q) x[r][q] = s[(j + q * LR) * BS + b];
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+                // w = exp(2 pi i k / (NS R)); twiddle of input q is w^q
+                float2 w[R];
+                w[1] = tw_lookup<L>(T, k * (L / (NS * R)));
 #pragma unroll
-                for (int q = 1; q < R; ++q) {
-                    // exp(2 pi i q k / (NS R)) = hi[(q k L/(NS R)) << (12 - LOGL)]
-                    const int e = (q * k * (L / (NS * R))) << (12 - LOGL);
-                    x[r][q] = cmul(x[r][q], tw.hi[e]);
+                for (int q = 2; q < R; ++q) {
+                    const int h = (q & (q - 1)) ? (1 << ilog2_c(q)) : (q >> 1);
+                    w[q] = cmul(w[h], w[q - h]);
                 }
+#pragma unroll
+                for (int q = 1; q < R; ++q) x[r][q] = cmul(x[r][q], w[q]);
             }
             dft_reg<R>(x[r]);
         }
@@ -115,7 +189,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, const Twiddles& tw, in
         if (NBUT % NT == 0 || beta < NBUT) {
             const int b = beta % B;
             const int j = beta / B;
-            const int k = j & (NS - 1);
+            const int k = j % NS;
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) s[(base + q * NS) * BS + b] = x[r][q];
@@ -126,23 +200,22 @@ __device__ __forceinline__ void stockham_stage(float2* s, const Twiddles& tw, in
 
 template <int L, int B, int BS, int NS, int NT>
 struct LdsFftStages {
-    static __device__ __forceinline__ void run(float2* s, const Twiddles& tw, int tid) {
+    static __device__ __forceinline__ void run(float2* s, const TwTable<L>& T, int tid) {
         if constexpr (NS < L) {
-            constexpr int REM = L / NS;
-            constexpr int R = REM >= 16 ? 16 : REM;
-            stockham_stage<L, B, BS, R, NS, NT>(s, tw, tid);
-            LdsFftStages<L, B, BS, NS * R, NT>::run(s, tw, tid);
+            constexpr int R = next_radix(L / NS);
+            stockham_stage<L, B, BS, R, NS, NT>(s, T, tid);
+            LdsFftStages<L, B, BS, NS * R, NT>::run(s, T, tid);
         }
     }
 };
 
-// Full in-place FFT (natural order in, natural order out) of B sequences of
-// length L (power of two, 2 <= L <= 4096) stored at s[i*BS + b].
-// Caller must __syncthreads() before (data written) -- returns synchronised.
+// Full in-place FFT (natural order in and out) of B sequences of length L
+// stored at s[i*BS + b].  The caller has synchronised after writing s and T;
+// returns synchronised.
 template <int L, int B, int BS, int NT>
-__device__ __forceinline__ void lds_fft(float2* s, const Twiddles& tw, int tid) {
-    static_assert(L >= 2 && L <= 4096 && (L & (L - 1)) == 0, "L must be 2^k <= 4096");
-    LdsFftStages<L, B, BS, 1, NT>::run(s, tw, tid);
+__device__ __forceinline__ void lds_fft(float2* s, const TwTable<L>& T, int tid) {
+    static_assert(L >= 2 && L <= 4096 && smooth235(L), "L must be 2^a 3^b 5^c <= 4096");
+    LdsFftStages<L, B, BS, 1, NT>::run(s, T, tid);
 }
 
 // padded LDS row stride for B interleaved sequences (odd stride avoids bank

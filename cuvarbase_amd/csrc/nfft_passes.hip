@@ -1,8 +1,8 @@
 // The two fused passes of the batched NFFT-GLS pipeline.
 //
-// Grid of n = n1*n2 cells, cell u = n2*u1 + u2 (u1 row, u2 column).  The
-// unnormalised + sign transform G[l] = sum_u g[u] exp(2 pi i l u / n) is done
-// four-step:
+// Grid of n = n1*n2 cells (n2 = 2^p2), cell u = n2*u1 + u2 (u1 row, u2
+// column).  The unnormalised + sign transform G[l] = sum_u g[u] exp(2 pi i l u/n)
+// is done four-step:
 //   pass 1  per column u2:  A[k1,u2] = exp(2 pi i u2 k1/n) sum_u1 g[n2 u1+u2] exp(2 pi i u1 k1/n1)
 //   pass 2  per row k1:     G[k1 + n1 k2] = sum_u2 A[k1,u2] exp(2 pi i u2 k2/n2)
 // Pass 1 builds its columns directly in LDS by Gaussian spreading (the work of
@@ -21,21 +21,24 @@ namespace gls {
 
 namespace {
 
-constexpr int NT = 256;
+constexpr int NT1 = 256;
 
 // ---------------------------------------------------------------------------
 // pass 1: spread + column FFT + inter-pass twiddle
 // ---------------------------------------------------------------------------
-template <int P1, int C1>
-__global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__ geom,
-                                                   const int32_t* __restrict__ curves,
-                                                   int ngrids, int p2, PointArrays pa,
-                                                   const int32_t* __restrict__ bkt,
-                                                   float2* __restrict__ grids, int m,
-                                                   Twiddles tw) {
-    constexpr int N1 = 1 << P1;
+template <int N1>
+__global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict__ geom,
+                                                    const int32_t* __restrict__ curves,
+                                                    int ngrids, int p2, PointArrays pa,
+                                                    const int32_t* __restrict__ bkt,
+                                                    float2* __restrict__ grids, int m) {
+    constexpr int C1 = pass1_cols(N1);
     constexpr int BS = lds_stride(C1);
+    constexpr int NH = (N1 + 63) / 64;
     __shared__ float2 tile[N1 * BS];
+    __shared__ TwTable<N1> T;
+    __shared__ float2 ehi[C1 * NH];   // exp(2 pi i u2 64 kh / n)
+    __shared__ float2 elo[C1 * 64];   // exp(2 pi i u2 kl / n)
     const int tid = threadIdx.x;
     const int n2 = 1 << p2;
     const int nb = n2 / C1;
@@ -44,10 +47,18 @@ __global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__
     const int gi = (bid / nb) % ngrids;
     const int ci = curves[bid / (nb * ngrids)];
     const CurveGeom& G = geom[ci];
-    const int p = G.p;
-    const int n = 1 << p;
+    const int n = G.n;
+    const int a = jb * C1;
 
-    for (int i = tid; i < N1 * BS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    for (int i = tid; i < N1 * BS; i += NT1) tile[i] = make_float2(0.f, 0.f);
+    build_twiddles<N1, NT1>(T, tid);
+    for (int i = tid; i < C1 * (NH + 64); i += NT1) {
+        const int jj = i / (NH + 64);
+        const int r = i - jj * (NH + 64);
+        const int64_t u2 = a + jj;
+        if (r < 64) elo[jj * 64 + r] = expi_ratio(u2 * r, n);
+        else ehi[jj * NH + (r - 64)] = expi_ratio(u2 * 64 * (r - 64), n);
+    }
     __syncthreads();
 
     // point set: buckets jb-nbk .. jb (cyclic), at most two contiguous ranges
@@ -73,9 +84,8 @@ __global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__
     const float2* Va = (gi == 0) ? pa.v0 : (gi == 1 ? pa.v1 : pa.v2);
     const float binv = (float)(1.4426950408889634 / G.b);   // log2(e)/b
     const float norm = (float)(1.0 / sqrt(M_PI * G.b));
-    const int a = jb * C1;
     const int two_m = 2 * m;
-    for (int q = tid; q < total; q += NT) {
+    for (int q = tid; q < total; q += NT1) {
         const int idx = q < len0 ? s0 + q : s1 + (q - len0);
         const int64_t o = G.pts + idx;
         const int U = Ua[o];
@@ -98,15 +108,15 @@ __global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__
     }
     __syncthreads();
 
-    lds_fft<N1, C1, BS, NT>(tile, tw, tid);
+    lds_fft<N1, C1, BS, NT1>(tile, T, tid);
 
     float2* out = grids + G.grid + (int64_t)gi * n;
-    for (int i = tid; i < N1 * C1; i += NT) {
-        const int k1 = i & (N1 - 1);
-        const int jj = i >> P1;
-        const int u2 = a + jj;
-        const float2 w = twiddle_pow2(tw, (uint32_t)u2 * (uint32_t)k1, p);
-        out[(int64_t)u2 * N1 + k1] = cmul(tile[k1 * BS + jj], w);
+    for (int i = tid; i < N1 * C1; i += NT1) {
+        const int k1 = i % N1;
+        const int jj = i / N1;
+        const float2 eh = ehi[jj * NH + (k1 >> 6)];
+        const float2 w = (k1 & 63) ? cmul(eh, elo[jj * 64 + (k1 & 63)]) : eh;
+        out[(int64_t)(a + jj) * N1 + k1] = cmul(tile[k1 * BS + jj], w);
     }
 }
 
@@ -114,28 +124,29 @@ __global__ __launch_bounds__(NT) void pass1_kernel(const CurveGeom* __restrict__
 // pass 2: row FFT + deconvolution + GLS assembly (or NFFT output)
 // ---------------------------------------------------------------------------
 // OUT: 0 = f32 power, 1 = f64 power, 2 = complex64 NFFT (single grid)
+template <int OUT>
+constexpr int pass2_threads() { return OUT == 2 ? 128 : 192; }
+
 template <int P2, int OUT>
-__global__ __launch_bounds__(NT) void pass2_kernel(const CurveGeom* __restrict__ geom,
-                                                   const CurveState* __restrict__ state,
-                                                   const int32_t* __restrict__ curves,
-                                                   int p1, const float2* __restrict__ grids,
-                                                   PlanParams pp, void* __restrict__ out,
-                                                   Twiddles tw) {
+__global__ __launch_bounds__(pass2_threads<OUT>()) void pass2_kernel(
+    const CurveGeom* __restrict__ geom, const CurveState* __restrict__ state,
+    const int32_t* __restrict__ curves, int n1, const float2* __restrict__ grids,
+    PlanParams pp, void* __restrict__ out) {
+    constexpr int NT = pass2_threads<OUT>();
     constexpr int N2 = 1 << P2;
     constexpr int CR = pass2_rows(P2);
     constexpr int NG = (OUT == 2) ? 1 : 3;
     constexpr int B = NG * CR;
     constexpr int BS = lds_stride(B);
     __shared__ float2 tile[N2 * BS];
+    __shared__ TwTable<N2> T;
     const int tid = threadIdx.x;
-    const int n1 = 1 << p1;
-    const int nrb = n1 / CR;
+    const int nrb = (n1 + CR - 1) / CR;   // the last row block may be partial
     const int bid = blockIdx.x;
     const int rb = bid % nrb;
     const int ci = curves[bid / nrb];
     const CurveGeom& G = geom[ci];
-    const int p = G.p;
-    const int n = 1 << p;
+    const int n = G.n;
     const int r0 = rb * CR;
 
     const float2* src = grids + G.grid;
@@ -144,11 +155,13 @@ __global__ __launch_bounds__(NT) void pass2_kernel(const CurveGeom* __restrict__
         const int u2 = i / B;
         const int gi = bb / CR;
         const int r = bb - gi * CR;
-        tile[u2 * BS + bb] = src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r];
+        tile[u2 * BS + bb] = (r0 + r < n1) ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
+                                           : make_float2(0.f, 0.f);
     }
+    build_twiddles<N2, NT>(T, tid);
     __syncthreads();
 
-    lds_fft<N2, B, BS, NT>(tile, tw, tid);
+    lds_fft<N2, B, BS, NT>(tile, T, tid);
 
     const int K = G.nf;
     const int kc = G.kc;
@@ -158,6 +171,7 @@ __global__ __launch_bounds__(NT) void pass2_kernel(const CurveGeom* __restrict__
     for (int i = tid; i < N2 * CR; i += NT) {
         const int r = i % CR;
         const int k2 = i / CR;
+        if (r0 + r >= n1) continue;
         const int l = r0 + r + n1 * k2;
         int lp;
         if (l < K - kc) lp = l;
@@ -185,85 +199,78 @@ __global__ __launch_bounds__(NT) void pass2_kernel(const CurveGeom* __restrict__
     }
 }
 
-template <int P1>
-hipError_t pass1_dispatch(int p, const CurveGeom* d_geom, const int32_t* curves, int ncurves,
-                          int ngrids, PointArrays pa, const int32_t* bkt, float2* grids, int m,
-                          Twiddles tw, hipStream_t s) {
-    constexpr int C1 = pass1_cols(P1);
-    const int p2 = p - P1;
+template <int N1>
+hipError_t pass1_dispatch(const CurveGeom* d_geom, const int32_t* curves, int ncurves,
+                          int ngrids, int p2, PointArrays pa, const int32_t* bkt,
+                          float2* grids, int m, hipStream_t s) {
+    constexpr int C1 = pass1_cols(N1);
     const int nb = (1 << p2) / C1;
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
-    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pass1_kernel<P1, C1>), dim3((unsigned)nblocks), dim3(NT), 0, s, d_geom,
-                       curves, ngrids, p2, pa, bkt, grids, m, tw);
+    if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, d_geom,
+                       curves, ngrids, p2, pa, bkt, grids, m);
     return hipGetLastError();
 }
 
 template <int P2, int OUT>
-hipError_t pass2_dispatch(int p1, const CurveGeom* d_geom, const CurveState* state,
+hipError_t pass2_dispatch(int n1, const CurveGeom* d_geom, const CurveState* state,
                           const int32_t* curves, int ncurves, const float2* grids,
-                          const PlanParams& pp, void* out, Twiddles tw, hipStream_t s) {
+                          const PlanParams& pp, void* out, hipStream_t s) {
     constexpr int CR = pass2_rows(P2);
-    const int64_t nblocks = (int64_t)ncurves * ((1 << p1) / CR);
+    const int64_t nblocks = (int64_t)ncurves * ((n1 + CR - 1) / CR);
     if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pass2_kernel<P2, OUT>), dim3((unsigned)nblocks), dim3(NT), 0, s, d_geom,
-                       state, curves, p1, grids, pp, out, tw);
+    hipLaunchKernelGGL((pass2_kernel<P2, OUT>), dim3((unsigned)nblocks),
+                       dim3(pass2_threads<OUT>()), 0, s, d_geom, state, curves, n1, grids, pp,
+                       out);
     return hipGetLastError();
 }
 
 template <int OUT>
-hipError_t pass2_switch(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+hipError_t pass2_switch(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
                         const int32_t* curves, int ncurves, const float2* grids,
-                        const PlanParams& pp, void* out, Twiddles tw, hipStream_t s) {
-    switch (p - p1) {
-        case 6: return pass2_dispatch<6, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 7: return pass2_dispatch<7, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 8: return pass2_dispatch<8, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 9: return pass2_dispatch<9, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 10: return pass2_dispatch<10, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 11: return pass2_dispatch<11, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
-        case 12: return pass2_dispatch<12, OUT>(p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+                        const PlanParams& pp, void* out, hipStream_t s) {
+    switch (p2) {
+        case 6: return pass2_dispatch<6, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 7: return pass2_dispatch<7, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 8: return pass2_dispatch<8, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 9: return pass2_dispatch<9, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 10: return pass2_dispatch<10, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 11: return pass2_dispatch<11, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        case 12: return pass2_dispatch<12, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_pass1(int p, int p1, int c1, const CurveGeom* d_geom, const int32_t* curves,
-                        int ncurves, int ngrids, PointArrays pa, const int32_t* bkt,
-                        float2* grids, int m, Twiddles tw, hipStream_t s) {
+hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves, int ncurves,
+                        int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
+                        int m, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
-    if (c1 != pass1_cols(p1)) return hipErrorInvalidValue;
-    switch (p1) {
-        case 4: return pass1_dispatch<4>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 5: return pass1_dispatch<5>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 6: return pass1_dispatch<6>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 7: return pass1_dispatch<7>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 8: return pass1_dispatch<8>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 9: return pass1_dispatch<9>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 10: return pass1_dispatch<10>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 11: return pass1_dispatch<11>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
-        case 12: return pass1_dispatch<12>(p, d_geom, curves, ncurves, ngrids, pa, bkt, grids, m, tw, s);
+    switch (n1) {
+#define GLS_PASS1_CASE(v) \
+    case v: return pass1_dispatch<v>(d_geom, curves, ncurves, ngrids, p2, pa, bkt, grids, m, s);
+        GLS_PASS1_SIZES(GLS_PASS1_CASE)
+#undef GLS_PASS1_CASE
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_pass2_lomb(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
                              const int32_t* curves, int ncurves, const float2* grids,
-                             const PlanParams& pp, void* power, int out_f64, Twiddles tw,
-                             hipStream_t s) {
+                             const PlanParams& pp, void* power, int out_f64, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
     if (out_f64)
-        return pass2_switch<1>(p, p1, d_geom, state, curves, ncurves, grids, pp, power, tw, s);
-    return pass2_switch<0>(p, p1, d_geom, state, curves, ncurves, grids, pp, power, tw, s);
+        return pass2_switch<1>(p2, n1, d_geom, state, curves, ncurves, grids, pp, power, s);
+    return pass2_switch<0>(p2, n1, d_geom, state, curves, ncurves, grids, pp, power, s);
 }
 
-hipError_t launch_pass2_nfft(int p, int p1, const CurveGeom* d_geom, const CurveState* state,
+hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
                              const int32_t* curves, int ncurves, const float2* grids,
-                             float2* out, Twiddles tw, hipStream_t s) {
+                             float2* out, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
     PlanParams pp{};
-    return pass2_switch<2>(p, p1, d_geom, state, curves, ncurves, grids, pp, out, tw, s);
+    return pass2_switch<2>(p2, n1, d_geom, state, curves, ncurves, grids, pp, out, s);
 }
 
 }  // namespace gls

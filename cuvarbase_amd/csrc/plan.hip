@@ -22,7 +22,7 @@ struct gls_plan {
     std::vector<gls::CurveGeom> geom;
     gls::CurveGeom* d_geom = nullptr;
     struct Bucket {
-        int p = 0, p1 = 0, c1 = 0;
+        int n1 = 0, p2 = 0, c1 = 0;
         int64_t first = 0;  // offset into d_bucket_curves
         int count = 0;
     };
@@ -39,7 +39,7 @@ struct gls_plan {
     // optional per-stage timing (events recorded between launches)
     struct Stage {
         int kind;      // 0 prep, 1 pass 1, 2 pass 2, 3 direct sums
-        int log2n;     // grid size of the bucket (0 for prep / direct)
+        int grid;      // grid size n of the bucket (0 for prep / direct)
         double bytes;  // algorithmic bytes moved by the stage
     };
     std::vector<Stage> stages;
@@ -61,45 +61,79 @@ struct Layout {
     }
 };
 
-int next_log2(double v) {
-    int p = 0;
-    while ((double)(1ull << p) < v && p < 62) ++p;
-    return p;
+int radix_stages(int L) {
+    int st = 0, rem = L;
+    const int radices[] = {16, 8, 4, 2, 5, 3};
+    while (rem > 1) {
+        for (int r : radices)
+            if (rem % r == 0) {
+                rem /= r;
+                ++st;
+                break;
+            }
+    }
+    return st;
 }
 
 double gauss_b(double sigma, int m) {
-    // b = 2 sigma m / ((2 sigma - 1) pi)  (cuvarbase/cunfft.py:56-57)
+    // b = 2 sigma m / ((2 sigma - 1) pi)  (cuvarbase/cunfft.py:56-57); for the
+    // centred band this balances the alias term exp(-pi^2 b (1 - 1/sigma))
+    // against the deconvolution-amplified truncation exp(-m^2/b + pi^2 b/(4 sigma^2))
     return 2.0 * sigma * (double)m / ((2.0 * sigma - 1.0) * M_PI);
+}
+
+// smallest-cost grid n = n1 * 2^p2 >= need from the size catalogue
+bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
+    double best = 0.0;
+    bool found = false;
+    if (need < kMinGrid) need = kMinGrid;
+    for (int p2 = kMinLog2Row; p2 <= kMaxLog2Row; ++p2) {
+        const int64_t n2 = 1ll << p2;
+        for (int i = 0; i < kNumPass1Sizes; ++i) {
+            const int n1 = kPass1Sizes[i];
+            const int64_t n = (int64_t)n1 * n2;
+            if (n < need || n > kMaxGrid) continue;
+            if (2 * m + pass1_cols(n1) > n2) continue;
+            const double cost =
+                (double)n * (1.0 + 0.12 * (radix_stages(n1) + radix_stages((int)n2)));
+            if (!found || cost < best) {
+                best = cost;
+                *n1_out = n1;
+                *p2_out = p2;
+                found = true;
+            }
+        }
+    }
+    return found;
 }
 
 // common planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft)
 int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts) {
     const int m = P->pp.m;
-    std::map<int, std::vector<int32_t>> by_p;
+    std::map<std::pair<int, int>, std::vector<int32_t>> by_size;
     int64_t bkt_ints = 0, grid_elems = 0;
     for (int i = 0; i < P->ncurves; ++i) {
         CurveGeom& G = P->geom[i];
-        const int p = std::max(kMinLog2Grid, next_log2(std::ceil(P->sigma * (double)G.nf)));
-        if (p > kMaxLog2Grid)
+        const int64_t need = (int64_t)std::ceil(P->sigma * (double)G.nf);
+        int n1 = 0, p2 = 0;
+        if (!choose_grid(need, m, &n1, &p2))
             return fail(GLS_E_UNSUPPORTED,
-                        "grid of curve " + std::to_string(i) + " exceeds 2^24 cells (nf=" +
-                            std::to_string(G.nf) + ")");
-        const int p1 = pass1_log2_len(p);
-        const int c1 = pass1_cols(p1);
-        const int n2 = 1 << (p - p1);
-        if (2 * m + c1 > n2)
-            return fail(GLS_E_UNSUPPORTED, "m too large for the grid of curve " + std::to_string(i));
-        const double n = (double)(1ll << p);
-        G.p = p;
-        G.p1 = p1;
+                        "no grid for curve " + std::to_string(i) + " (nf=" +
+                            std::to_string(G.nf) + ", sigma*nf must be <= 2^24)");
+        const int c1 = pass1_cols(n1);
+        const int n2 = 1 << p2;
+        const int64_t n = (int64_t)n1 * n2;
+        G.n = (int32_t)n;
+        G.n1 = n1;
+        G.p2 = p2;
         G.c1 = c1;
         G.kc = G.nf / 2;
-        G.b = gauss_b(n / (double)G.nf, m);
+        G.b = gauss_b((double)n / (double)G.nf, m);
         G.bkt = bkt_ints;
         bkt_ints += 2 * (n2 / c1 + 1);
         G.grid = grid_elems;
-        grid_elems += (int64_t)ngrids << p;
-        by_p[p].push_back(i);
+        grid_elems += (int64_t)ngrids * n;
+        by_size[{n1, p2}].push_back(i);
     }
     P->off_U1 = L.take(sizeof(int32_t) * total_pts);
     P->off_fr1 = L.take(sizeof(float) * total_pts);
@@ -113,11 +147,11 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts) {
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
     std::vector<int32_t> flat;
-    for (auto& kv : by_p) {
+    for (auto& kv : by_size) {
         gls_plan::Bucket B;
-        B.p = kv.first;
-        B.p1 = pass1_log2_len(B.p);
-        B.c1 = pass1_cols(B.p1);
+        B.n1 = kv.first.first;
+        B.p2 = kv.first.second;
+        B.c1 = pass1_cols(B.n1);
         B.first = (int64_t)flat.size();
         B.count = (int)kv.second.size();
         flat.insert(flat.end(), kv.second.begin(), kv.second.end());
@@ -181,13 +215,13 @@ void build_stage_list(gls_plan* P) {
             double b1 = 0.0, b2 = 0.0;
             // curves of the bucket are contiguous in bucket order; recompute from geometry
             for (auto& G : P->geom) {
-                if (G.p != B.p) continue;
-                const double n = (double)(1ll << G.p);
+                if (G.n1 != B.n1 || G.p2 != B.p2) continue;
+                const double n = (double)G.n;
                 b1 += ngr * (16.0 * G.n_obs + 8.0 * n);
                 b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : out_b) * G.nf;
             }
-            P->stages.push_back({1, B.p, b1});
-            P->stages.push_back({2, B.p, b2});
+            P->stages.push_back({1, B.n1 << B.p2, b1});
+            P->stages.push_back({2, B.n1 << B.p2, b2});
         }
     } else {
         double b = 0.0;
@@ -254,15 +288,14 @@ int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, voi
     GLS_HIP_TRY(mark());
     GLS_HIP_TRY(launch_prep(P->d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, 0, s));
     GLS_HIP_TRY(mark());
-    const Twiddles tw = device_twiddles();
     if (P->pp.method == GLS_METHOD_NFFT) {
         for (const auto& B : P->buckets) {
             const int32_t* curves = P->d_bucket_curves + B.first;
-            GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, P->d_geom, curves, B.count, 3, pa, bkt,
-                                     grids, P->pp.m, tw, s));
+            GLS_HIP_TRY(launch_pass1(B.n1, P->d_geom, curves, B.count, 3, B.p2, pa, bkt, grids,
+                                     P->pp.m, s));
             GLS_HIP_TRY(mark());
-            GLS_HIP_TRY(launch_pass2_lomb(B.p, B.p1, P->d_geom, state, curves, B.count, grids,
-                                          P->pp, power, out_f64, tw, s));
+            GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, P->d_geom, state, curves, B.count, grids,
+                                          P->pp, power, out_f64, s));
             GLS_HIP_TRY(mark());
         }
     } else {
@@ -423,7 +456,20 @@ int gls_plan_output_count(const gls_plan* plan, int64_t* count) {
 int gls_plan_grid_size(const gls_plan* plan, int32_t i, int64_t* n) {
     if (!plan || !n) return fail(GLS_E_INVALID, "NULL argument");
     if (i < 0 || i >= plan->ncurves) return fail(GLS_E_INVALID, "curve index out of range");
-    *n = (plan->pp.method == GLS_METHOD_NFFT) ? (1ll << plan->geom[i].p) : 0;
+    *n = (plan->pp.method == GLS_METHOD_NFFT) ? (int64_t)plan->geom[i].n : 0;
+    return GLS_OK;
+}
+
+int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n, int32_t* n1,
+                            int32_t* n2, double* b) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (i < 0 || i >= plan->ncurves) return fail(GLS_E_INVALID, "curve index out of range");
+    const CurveGeom& G = plan->geom[i];
+    const bool nfft = plan->pp.method == GLS_METHOD_NFFT;
+    if (n) *n = nfft ? (int64_t)G.n : 0;
+    if (n1) *n1 = nfft ? G.n1 : 0;
+    if (n2) *n2 = nfft ? (1 << G.p2) : 0;
+    if (b) *b = nfft ? G.b : 0.0;
     return GLS_OK;
 }
 
@@ -460,14 +506,13 @@ int gls_nfft_execute(gls_plan* plan, const double* t, const void* y, int32_t y_i
     GLS_HIP_TRY(launch_prep(plan->d_geom, plan->ncurves, state, t, y, y, pa, bkt, plan->pp,
                             y_is_f32 ? 1 : 0, s));
     GLS_HIP_TRY(mark());
-    const Twiddles tw = device_twiddles();
     for (const auto& B : plan->buckets) {
         const int32_t* curves = plan->d_bucket_curves + B.first;
-        GLS_HIP_TRY(launch_pass1(B.p, B.p1, B.c1, plan->d_geom, curves, B.count, 1, pa, bkt,
-                                 grids, plan->pp.m, tw, s));
+        GLS_HIP_TRY(launch_pass1(B.n1, plan->d_geom, curves, B.count, 1, B.p2, pa, bkt, grids,
+                                 plan->pp.m, s));
         GLS_HIP_TRY(mark());
-        GLS_HIP_TRY(launch_pass2_nfft(B.p, B.p1, plan->d_geom, state, curves, B.count, grids,
-                                      (float2*)out, tw, s));
+        GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, plan->d_geom, state, curves, B.count, grids,
+                                      (float2*)out, s));
         GLS_HIP_TRY(mark());
     }
     return GLS_OK;
@@ -502,13 +547,13 @@ int gls_plan_stage_count(const gls_plan* plan, int32_t* count) {
     return GLS_OK;
 }
 
-int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* log2n, float* ms,
+int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* grid, float* ms,
                          double* bytes, int32_t max) {
     if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
     const int ns = (int)plan->stages.size();
     for (int i = 0; i < ns && i < max; ++i) {
         if (kind) kind[i] = plan->stages[i].kind;
-        if (log2n) log2n[i] = plan->stages[i].log2n;
+        if (grid) grid[i] = plan->stages[i].grid;
         if (bytes) bytes[i] = plan->stages[i].bytes;
         if (ms) {
             ms[i] = 0.f;

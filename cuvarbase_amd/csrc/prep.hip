@@ -133,8 +133,8 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
     }
 
     // ---- phase B: cells, modulation and bucketing -------------------------
-    const int n = 1 << G.p;
-    const int p2 = G.p - G.p1;
+    const int n = G.n;
+    const int p2 = G.p2;
     const int n2 = 1 << p2;
     const int c1 = G.c1;
     const int nb = n2 / c1;

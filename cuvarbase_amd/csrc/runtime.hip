@@ -13,8 +13,6 @@ thread_local std::string g_err;
 std::mutex g_mu;
 bool g_init = false;
 int g_device = -1;
-float2* g_tw_hi = nullptr;
-float2* g_tw_lo = nullptr;
 }  // namespace
 
 void set_error(const std::string& msg) { g_err = msg; }
@@ -35,8 +33,6 @@ int require_init() {
     if (!g_init) return fail(GLS_E_NOT_INIT, "gls_init() has not been called");
     return GLS_OK;
 }
-
-Twiddles device_twiddles() { return Twiddles{g_tw_hi, g_tw_lo}; }
 
 }  // namespace gls
 
@@ -75,23 +71,6 @@ int gls_init(int device) {
     if (device < 0 || device >= n)
         return fail(GLS_E_INVALID, "device index out of range");
     GLS_HIP_TRY(hipSetDevice(device));
-    // resident twiddle tables, built in FP64 on the host
-    std::vector<float2> hi(4096), lo(4096);
-    for (int i = 0; i < 4096; ++i) {
-        const double a = 2.0 * M_PI * (double)i / 4096.0;
-        hi[i] = make_float2((float)std::cos(a), (float)std::sin(a));
-        const double b = 2.0 * M_PI * (double)i / 16777216.0;
-        lo[i] = make_float2((float)std::cos(b), (float)std::sin(b));
-    }
-    // exact values on the axes
-    hi[0] = make_float2(1.f, 0.f);
-    hi[1024] = make_float2(0.f, 1.f);
-    hi[2048] = make_float2(-1.f, 0.f);
-    hi[3072] = make_float2(0.f, -1.f);
-    GLS_HIP_TRY(hipMalloc(&g_tw_hi, 4096 * sizeof(float2)));
-    GLS_HIP_TRY(hipMalloc(&g_tw_lo, 4096 * sizeof(float2)));
-    GLS_HIP_TRY(hipMemcpy(g_tw_hi, hi.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
-    GLS_HIP_TRY(hipMemcpy(g_tw_lo, lo.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
     g_device = device;
     g_init = true;
     return GLS_OK;
@@ -100,9 +79,6 @@ int gls_init(int device) {
 int gls_shutdown(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_init) return GLS_OK;
-    hipFree(g_tw_hi);
-    hipFree(g_tw_lo);
-    g_tw_hi = g_tw_lo = nullptr;
     g_init = false;
     g_device = -1;
     return GLS_OK;

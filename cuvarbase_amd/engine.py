@@ -74,13 +74,23 @@ class Plan:
         _lib.call("gls_plan_grid_size", self.handle, C.c_int32(i), C.byref(v))
         return v.value
 
+    def geometry(self, i):
+        """(n, n1, n2, b) of curve i: grid size, four-step split, Gaussian width."""
+        n = C.c_int64(0)
+        n1 = C.c_int32(0)
+        n2 = C.c_int32(0)
+        b = C.c_double(0)
+        _lib.call("gls_plan_curve_geometry", self.handle, C.c_int32(i), C.byref(n), C.byref(n1),
+                  C.byref(n2), C.byref(b))
+        return n.value, n1.value, n2.value, b.value
+
     def enable_timing(self, enable=True):
         _lib.call("gls_plan_enable_timing", self.handle, C.c_int32(1 if enable else 0))
 
     STAGE_NAMES = {0: "prep", 1: "pass1", 2: "pass2", 3: "dirsum"}
 
     def stage_times(self):
-        """[(stage, log2n, ms, algorithmic bytes)] of the last timed execute
+        """[(stage, grid n, ms, algorithmic bytes)] of the last timed execute
         (call after the stream has completed)."""
         n = C.c_int32(0)
         _lib.call("gls_plan_stage_count", self.handle, C.byref(n))

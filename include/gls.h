@@ -96,8 +96,7 @@ typedef struct gls_plan gls_plan;   /* opaque */
 
 /* ---- context ------------------------------------------------------------ */
 /* Replaces the import-time context creation `import pycuda.autoprimaryctx`
- * (cuvarbase/__init__.py:2): selects `device` and builds the resident
- * twiddle tables.  Idempotent per device. */
+ * (cuvarbase/__init__.py:2): selects `device`.  Idempotent per device. */
 int gls_init(int device);
 int gls_shutdown(void);
 const char* gls_last_error(void);
@@ -133,7 +132,8 @@ int gls_memset_async(void* dst, int value, size_t bytes, void* stream);
 int gls_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- GLS periodogram ---------------------------------------------------- */
-/* Plans a batch: chooses each curve's grid size (n >= sigma*nf, FFT-friendly)
+/* Plans a batch: chooses each curve's grid size n >= sigma*nf of the form
+ * n1 * 2^p2 (n1 = 2^a 3^b 5^c from a fixed catalogue, 2^10 <= n <= 2^24)
  * and lays out the workspace.  Replaces LombScargleMemory.allocate +
  * NFFTMemory.allocate_grid + the two cuFFT plans per curve
  * (cuvarbase/lombscargle.py:129-181, cuvarbase/cunfft.py:82-92).
@@ -147,6 +147,10 @@ int gls_plan_workspace_bytes(const gls_plan* plan, size_t* bytes);
 int gls_plan_output_count(const gls_plan* plan, int64_t* count);
 /* Grid size chosen for curve i (0 for the direct method). */
 int gls_plan_grid_size(const gls_plan* plan, int32_t i, int64_t* n);
+/* Four-step split n = n1 * n2 and Gaussian width b (cells^2) of curve i
+ * (zeros for the direct method). */
+int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n,
+                            int32_t* n1, int32_t* n2, double* b);
 
 /* Runs the batch on `stream`: replaces lomb_scargle_async for every curve
  * (cuvarbase/lombscargle.py:579-689: nfft_adjoint_async twice + `lomb`, or
@@ -173,11 +177,11 @@ int gls_lomb_execute_f64(gls_plan* plan, const double* t, const void* a,
  * the launches of the plan's stages: prep, then per grid-size bucket pass 1
  * and pass 2 (or the direct-sum launch).  After the stream has completed,
  * gls_plan_stage_times reports for each stage its kind (0 prep, 1 pass 1,
- * 2 pass 2, 3 direct sums), log2 grid size, elapsed ms and the algorithmic
- * bytes it must move. */
+ * 2 pass 2, 3 direct sums), grid size n of its bucket, elapsed ms and the
+ * algorithmic bytes it must move. */
 int gls_plan_enable_timing(gls_plan* plan, int32_t enable);
 int gls_plan_stage_count(const gls_plan* plan, int32_t* count);
-int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* log2n,
+int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* grid,
                          float* ms, double* bytes, int32_t max);
 
 /* ---- adjoint NFFT (NFFTAsyncProcess.run, cuvarbase/cunfft.py:507-542) --- */

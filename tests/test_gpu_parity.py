@@ -83,13 +83,13 @@ def test_grid_size_sweep_against_oracle_nfft(nf):
     freqs = df * (1 + np.arange(nf))
     b = engine.LombBatch([(t, y, dy)], [freqs], sigma=2, m=8)
     P = b.run()[0].astype(np.float64)
-    ng = b.plan.grid_size(0)
-    s = ng / nf
-    Po = on.gls_nfft(t, y, dy, freqs, n=ng, m=8, b=2 * s * 8 / ((2 * s - 1) * np.pi))
+    ng, n1, n2, bw = b.plan.geometry(0)
+    assert ng == n1 * n2 and ng >= 2 * nf and ng == b.plan.grid_size(0)
+    Po = on.gls_nfft(t, y, dy, freqs, n=ng, m=8, b=bw)
     sel = rng.choice(nf, size=min(nf, 200), replace=False)
     Pd = og.gls_direct(t, y, dy, freqs[sel])
     e1, e2 = np.max(np.abs(P - Po)), np.max(np.abs(P[sel] - Pd))
-    print(f"nf={nf} n={ng}: vs oracle NFFT {e1:.3e}, vs direct {e2:.3e}")
+    print(f"nf={nf} n={ng}={n1}x{n2}: vs oracle NFFT {e1:.3e}, vs direct {e2:.3e}")
     assert e1 < 1e-5 and e2 < 1e-5
     assert np.argmax(P) == np.argmax(Po)
 
