@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--sigma", type=float, default=None,
                     help="NFFT oversampling (default: tuned for |dP| <= 1e-4)")
     ap.add_argument("--m", type=int, default=None)
+    ap.add_argument("--stream-requests", type=int, default=1500,
+                    help="requests per arrival rate in the serving leg (0: skip)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the GLS path "
@@ -194,6 +197,39 @@ def main():
         ev += len(freqs[i])
     plain = ev / (time.perf_counter() - t0) if nplain else None
 
+    # served path: continuous batching of host-resident requests arriving as a
+    # Poisson stream (PCIe transfers included), and the plain path on the same
+    # requests, one per run() + finish()
+    serving = None
+    if args.stream_requests > 0:
+        del batch, flat, P
+        _device.synchronize()
+        from cuvarbase_amd.serve import GLSServer
+
+        arr, sdata, sfreqs = synthetic.stream_requests(1.0, nreq=args.stream_requests)
+        srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m)
+        serving = {"requests_per_rate": args.stream_requests, "hbm_budget_gb": args.hbm_budget_gb,
+                   "rates": {}}
+        for rate in (100.0, 1000.0, 10000.0, float("inf")):
+            nreq = args.stream_requests if rate > 500 else min(args.stream_requests,
+                                                             int(rate * 10))
+            a = np.zeros(nreq) if math.isinf(rate) else arr[:nreq] / rate
+            st = srv.serve_stream(a, sdata[:nreq], sfreqs[:nreq])
+            srv.completed.clear()
+            serving["rates"]["burst" if math.isinf(rate) else f"{rate:g}/s"] = {
+                k: st[k] for k in ("requests", "evals", "wall_s", "evals_per_s", "p50_ms",
+                                   "p99_ms", "batches")}
+        t0 = time.perf_counter()
+        ev = 0
+        nps = min(100, len(sdata))
+        for i in range(nps):
+            proc.run([sdata[i]], freqs=[sfreqs[i]])
+            proc.finish()
+            ev += len(sfreqs[i])
+        serving["plain_evals_per_s"] = ev / (time.perf_counter() - t0)
+        best = max(v["evals_per_s"] for v in serving["rates"].values())
+        serving["served_vs_plain"] = best / serving["plain_evals_per_s"]
+
     # pipeline-level roofline: fixed algorithmic bytes B = 16N + 24G + 28nf
     nobs = np.array([len(d[0]) for d in data], dtype=np.float64)
     nfa = np.array([len(f) for f in freqs], dtype=np.float64)
@@ -246,6 +282,7 @@ def main():
         "latency_ms": {"p50": p50, "p99": p99},
         "parity": {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(idx)}",
                    "oracle": "FP64 NumPy NFFT sigma=4 m=12"},
+        "serving": serving,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
     }

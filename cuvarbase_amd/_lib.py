@@ -107,7 +107,7 @@ _PROTOS = {
     "gls_nfft_execute": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
     "gls_gaussian_grid": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                           C.c_double, C.c_double, C.c_double, _vp, _vp],
-    "gls_topk": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
+    "gls_topk": [_vp, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
     "gls_plan_enable_timing": [_vp, C.c_int32],
     "gls_plan_stage_count": [_vp, C.POINTER(C.c_int32)],
     "gls_plan_stage_times": [_vp, _vp, _vp, _vp, _vp, C.c_int32],

@@ -1,15 +1,22 @@
 // Batch planning and execution (C ABI).
 //
 // A plan fixes, for every curve of a batch, the grid size of its three NFFTs
-// (n = 2^p >= sigma*nf, at least 2^10), the four-step split n = n1*n2, the
-// Gaussian width b and the workspace layout.  Curves with the same n form a
-// size bucket; each bucket runs as two launches (pass 1 and pass 2) after one
-// prep launch for the whole batch.  This replaces the reference's per-curve
-// allocation, two cuFFT plans per curve and nine launches per curve
+// (n = n1 * 2^p2 >= sigma*nf from the size catalogue), the four-step split,
+// the Gaussian width b and the workspace layout.  Curves with the same split
+// form a size bucket; each bucket runs as two launches (pass 1 and pass 2)
+// after one prep launch for the whole batch.  This replaces the reference's
+// per-curve allocation, two cuFFT plans per curve and nine launches per curve
 // (cuvarbase/lombscargle.py:129-181, :579-689; cuvarbase/cunfft.py:82-92,
 // :163-308).
+//
+// A plan is a host object only: it owns no device memory.  Its per-curve
+// descriptors are copied (hipMemcpyAsync, on the execution stream) into the
+// first bytes of the caller's workspace at every execute, so plans can be
+// created and destroyed while other batches run without any device
+// allocation or the implicit synchronisation of hipFree.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 
 #include "gls_internal.hpp"
@@ -20,17 +27,17 @@ struct gls_plan {
     double sigma = 0.0;
     int ncurves = 0;
     std::vector<gls::CurveGeom> geom;
-    gls::CurveGeom* d_geom = nullptr;
     struct Bucket {
         int n1 = 0, p2 = 0, c1 = 0;
-        int64_t first = 0;  // offset into d_bucket_curves
+        int64_t first = 0;  // offset (curves) into the bucket curve list
         int count = 0;
     };
     std::vector<Bucket> buckets;
-    int32_t* d_bucket_curves = nullptr;
-    int32_t* d_tiles = nullptr;  // direct method: tile_curve[ntiles], tile_first[ntiles]
-    int ntiles = 0;
+    int ntiles = 0;         // direct method
     int rf = 1;
+    // host image of the descriptors uploaded into workspace[0 .. meta.size())
+    std::vector<char> meta;
+    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0;
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
@@ -65,12 +72,15 @@ int radix_stages(int L) {
     int st = 0, rem = L;
     const int radices[] = {16, 8, 4, 2, 5, 3};
     while (rem > 1) {
+        bool found = false;
         for (int r : radices)
             if (rem % r == 0) {
                 rem /= r;
                 ++st;
+                found = true;
                 break;
             }
+        if (!found) return 99;
     }
     return st;
 }
@@ -107,8 +117,9 @@ bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
     return found;
 }
 
-// common planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft)
-int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts) {
+// planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft)
+int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
+                       std::vector<int32_t>& bcurves) {
     const int m = P->pp.m;
     std::map<std::pair<int, int>, std::vector<int32_t>> by_size;
     int64_t bkt_ints = 0, grid_elems = 0;
@@ -146,33 +157,25 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts) {
     }
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
-    std::vector<int32_t> flat;
     for (auto& kv : by_size) {
         gls_plan::Bucket B;
         B.n1 = kv.first.first;
         B.p2 = kv.first.second;
         B.c1 = pass1_cols(B.n1);
-        B.first = (int64_t)flat.size();
+        B.first = (int64_t)bcurves.size();
         B.count = (int)kv.second.size();
-        flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+        bcurves.insert(bcurves.end(), kv.second.begin(), kv.second.end());
         P->buckets.push_back(B);
-    }
-    if (!flat.empty()) {
-        GLS_HIP_TRY(hipMalloc(&P->d_bucket_curves, sizeof(int32_t) * flat.size()));
-        GLS_HIP_TRY(hipMemcpy(P->d_bucket_curves, flat.data(), sizeof(int32_t) * flat.size(),
-                              hipMemcpyHostToDevice));
     }
     return GLS_OK;
 }
 
-int plan_direct_tiles(gls_plan* P) {
-    int64_t total_f = 0;
-    for (auto& G : P->geom) total_f += G.nf;
+int plan_direct_tiles(gls_plan* P, std::vector<int32_t>& tiles) {
     int rf = 8;
     while (rf > 1) {
-        int64_t tiles = 0;
-        for (auto& G : P->geom) tiles += (G.nf + 256 * rf - 1) / (256 * rf);
-        if (tiles >= 512) break;
+        int64_t nt = 0;
+        for (auto& G : P->geom) nt += (G.nf + 256 * rf - 1) / (256 * rf);
+        if (nt >= 512) break;
         rf >>= 1;
     }
     std::vector<int32_t> tc, tf;
@@ -183,17 +186,11 @@ int plan_direct_tiles(gls_plan* P) {
             tf.push_back(k);
         }
     }
-    if (tc.size() > 0x7fffffff) return fail(GLS_E_UNSUPPORTED, "too many direct-sum tiles");
+    if (tc.size() > 0x3fffffff) return fail(GLS_E_UNSUPPORTED, "too many direct-sum tiles");
     P->rf = rf;
     P->ntiles = (int)tc.size();
-    if (P->ntiles > 0) {
-        GLS_HIP_TRY(hipMalloc(&P->d_tiles, sizeof(int32_t) * 2 * tc.size()));
-        GLS_HIP_TRY(hipMemcpy(P->d_tiles, tc.data(), sizeof(int32_t) * tc.size(),
-                              hipMemcpyHostToDevice));
-        GLS_HIP_TRY(hipMemcpy(P->d_tiles + tc.size(), tf.data(), sizeof(int32_t) * tf.size(),
-                              hipMemcpyHostToDevice));
-    }
-    (void)total_f;
+    tiles = tc;
+    tiles.insert(tiles.end(), tf.begin(), tf.end());
     return GLS_OK;
 }
 
@@ -209,43 +206,47 @@ void build_stage_list(gls_plan* P) {
         prep += (double)G.n_obs * (in_bytes + rec);
     }
     P->stages.push_back({0, 0, prep});
-    const double out_b = 4.0;
     if (P->pp.method == GLS_METHOD_NFFT) {
         for (const auto& B : P->buckets) {
             double b1 = 0.0, b2 = 0.0;
-            // curves of the bucket are contiguous in bucket order; recompute from geometry
             for (auto& G : P->geom) {
                 if (G.n1 != B.n1 || G.p2 != B.p2) continue;
                 const double n = (double)G.n;
                 b1 += ngr * (16.0 * G.n_obs + 8.0 * n);
-                b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : out_b) * G.nf;
+                b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : 4.0) * G.nf;
             }
             P->stages.push_back({1, B.n1 << B.p2, b1});
             P->stages.push_back({2, B.n1 << B.p2, b2});
         }
     } else {
         double b = 0.0;
-        for (auto& G : P->geom) b += 16.0 * G.n_obs + out_b * G.nf;
+        for (auto& G : P->geom) b += 16.0 * G.n_obs + 4.0 * G.nf;
         P->stages.push_back({3, 0, b});
     }
 }
 
-int finish_plan(gls_plan* P, Layout& L) {
-    build_stage_list(P);
-    P->ws_bytes = std::max<size_t>(L.cur, 256);
-    if (!P->geom.empty()) {
-        GLS_HIP_TRY(hipMalloc(&P->d_geom, sizeof(CurveGeom) * P->geom.size()));
-        GLS_HIP_TRY(hipMemcpy(P->d_geom, P->geom.data(), sizeof(CurveGeom) * P->geom.size(),
-                              hipMemcpyHostToDevice));
-    }
-    return GLS_OK;
+// lays out [geom | bucket curves | tiles] at the start of the workspace
+void reserve_meta(gls_plan* P, Layout& L, size_t nbcurves, size_t ntiles_ints) {
+    P->off_geom = L.take(sizeof(CurveGeom) * P->geom.size());
+    P->off_bcurves = L.take(sizeof(int32_t) * nbcurves);
+    P->off_tiles = L.take(sizeof(int32_t) * ntiles_ints);
+}
+
+void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
+               const std::vector<int32_t>& tiles) {
+    P->meta.assign(P->off_tiles + sizeof(int32_t) * tiles.size(), 0);
+    if (!P->geom.empty())
+        std::memcpy(P->meta.data() + P->off_geom, P->geom.data(),
+                    sizeof(CurveGeom) * P->geom.size());
+    if (!bcurves.empty())
+        std::memcpy(P->meta.data() + P->off_bcurves, bcurves.data(),
+                    sizeof(int32_t) * bcurves.size());
+    if (!tiles.empty())
+        std::memcpy(P->meta.data() + P->off_tiles, tiles.data(), sizeof(int32_t) * tiles.size());
 }
 
 void destroy_plan(gls_plan* P) {
     if (!P) return;
-    if (P->d_geom) hipFree(P->d_geom);
-    if (P->d_bucket_curves) hipFree(P->d_bucket_curves);
-    if (P->d_tiles) hipFree(P->d_tiles);
     for (auto e : P->events) hipEventDestroy(e);
     delete P;
 }
@@ -266,16 +267,16 @@ PointArrays point_arrays(const gls_plan* P, void* ws) {
     return pa;
 }
 
-int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, void* power,
-                 int out_f64, void* ws, void* stream) {
-    int rc = require_init();
-    if (rc) return rc;
-    if (!P) return fail(GLS_E_INVALID, "plan is NULL");
-    if (P->kind != 0) return fail(GLS_E_INVALID, "plan is not a Lomb-Scargle plan");
-    if (P->ncurves == 0) return GLS_OK;
-    if (!t || !a || !b || !power || !ws) return fail(GLS_E_INVALID, "NULL device pointer");
-    hipStream_t s = (hipStream_t)stream;
+// The whole launch sequence of a plan on stream s.
+int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* out,
+             int out_f64, int y_is_f32, void* ws, hipStream_t s) {
     char* base = (char*)ws;
+    if (!P->meta.empty())
+        GLS_HIP_TRY(hipMemcpyAsync(base, P->meta.data(), P->meta.size(), hipMemcpyHostToDevice,
+                                   s));
+    const CurveGeom* d_geom = (const CurveGeom*)(base + P->off_geom);
+    const int32_t* d_bcurves = (const int32_t*)(base + P->off_bcurves);
+    const int32_t* d_tiles = (const int32_t*)(base + P->off_tiles);
     CurveState* state = (CurveState*)(base + P->off_state);
     PointArrays pa = point_arrays(P, ws);
     int32_t* bkt = (int32_t*)(base + P->off_bkt);
@@ -285,25 +286,54 @@ int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, voi
         if (!P->timing) return hipSuccess;
         return hipEventRecord(P->events[ev++], s);
     };
+    const int ngrids = (P->kind == 1) ? 1 : 3;
     GLS_HIP_TRY(mark());
-    GLS_HIP_TRY(launch_prep(P->d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, 0, s));
+    GLS_HIP_TRY(launch_prep(d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, y_is_f32, s));
     GLS_HIP_TRY(mark());
     if (P->pp.method == GLS_METHOD_NFFT) {
         for (const auto& B : P->buckets) {
-            const int32_t* curves = P->d_bucket_curves + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, P->d_geom, curves, B.count, 3, B.p2, pa, bkt, grids,
+            const int32_t* curves = d_bcurves + B.first;
+            GLS_HIP_TRY(launch_pass1(B.n1, d_geom, curves, B.count, ngrids, B.p2, pa, bkt, grids,
                                      P->pp.m, s));
             GLS_HIP_TRY(mark());
-            GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, P->d_geom, state, curves, B.count, grids,
-                                          P->pp, power, out_f64, s));
+            if (P->kind == 1)
+                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, d_geom, state, curves, B.count, grids,
+                                              (float2*)out, s));
+            else
+                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, d_geom, state, curves, B.count, grids,
+                                              P->pp, out, out_f64, s));
             GLS_HIP_TRY(mark());
         }
     } else {
-        GLS_HIP_TRY(launch_dirsum(P->d_geom, state, P->d_tiles, P->d_tiles + P->ntiles,
-                                  P->ntiles, P->rf, pa, P->pp, power, out_f64, s));
+        GLS_HIP_TRY(launch_dirsum(d_geom, state, d_tiles, d_tiles + P->ntiles, P->ntiles, P->rf,
+                                  pa, P->pp, out, out_f64, s));
         GLS_HIP_TRY(mark());
     }
     return GLS_OK;
+}
+
+int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, void* power,
+                 int out_f64, void* ws, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!P) return fail(GLS_E_INVALID, "plan is NULL");
+    if (P->kind != 0) return fail(GLS_E_INVALID, "plan is not a Lomb-Scargle plan");
+    if (P->ncurves == 0) return GLS_OK;
+    if (!t || !a || !b || !power || !ws) return fail(GLS_E_INVALID, "NULL device pointer");
+    return run_plan(P, t, a, b, power, out_f64, 0, ws, (hipStream_t)stream);
+}
+
+int finish_plan(gls_plan* P, Layout& L) {
+    build_stage_list(P);
+    P->ws_bytes = std::max<size_t>(L.cur, 256);
+    return GLS_OK;
+}
+
+int check_curve(int64_t obs, int32_t n_obs, int32_t nf, double f0, double df, int64_t out) {
+    return (n_obs < 0 || nf < 1 || obs < 0 || out < 0 || !std::isfinite(f0) ||
+            !std::isfinite(df) || !(df > 0.0))
+               ? 1
+               : 0;
 }
 
 }  // namespace
@@ -348,8 +378,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     int64_t pts = 0;
     for (int i = 0; i < ncurves; ++i) {
         const gls_curve& c = curves[i];
-        if (c.n_obs < 0 || c.nf < 1 || c.obs_offset < 0 || c.out_offset < 0 ||
-            !std::isfinite(c.f0) || !std::isfinite(c.df) || !(c.df > 0.0)) {
+        if (check_curve(c.obs_offset, c.n_obs, c.nf, c.f0, c.df, c.out_offset)) {
             destroy_plan(P);
             return fail(GLS_E_INVALID, "invalid curve descriptor " + std::to_string(i));
         }
@@ -367,17 +396,39 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         pts += c.n_obs;
         P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
     }
+    std::vector<int32_t> bcurves, tiles;
+    if (prm.method == GLS_METHOD_DIRECT) rc = plan_direct_tiles(P, tiles);
+    if (rc != GLS_OK) {
+        destroy_plan(P);
+        return rc;
+    }
+    // bucket curve lists need the geometry first: plan into a scratch layout,
+    // then lay out [meta | state | data] for real
     Layout L;
-    P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
     if (prm.method == GLS_METHOD_NFFT) {
-        rc = plan_nfft_geometry(P, 3, L, pts);
+        Layout probe;
+        rc = plan_nfft_geometry(P, 3, probe, pts, bcurves);
+        if (rc != GLS_OK) {
+            destroy_plan(P);
+            return rc;
+        }
+        reserve_meta(P, L, bcurves.size(), 0);
+        P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
+        // re-take the data regions after the meta and state regions
+        const size_t shift = L.cur;
+        for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_v1, &P->off_U2,
+                          &P->off_fr2, &P->off_v2, &P->off_bkt, &P->off_grid})
+            *o += shift;
+        L.cur += probe.cur;
     } else {
+        reserve_meta(P, L, 0, tiles.size());
+        P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
         P->off_w = L.take(sizeof(float) * pts);
         P->off_yw = L.take(sizeof(float) * pts);
         P->off_dt = L.take(sizeof(double) * pts);
-        rc = plan_direct_tiles(P);
     }
-    if (rc == GLS_OK) rc = finish_plan(P, L);
+    fill_meta(P, bcurves, tiles);
+    rc = finish_plan(P, L);
     if (rc != GLS_OK) {
         destroy_plan(P);
         return rc;
@@ -407,8 +458,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     int64_t pts = 0;
     for (int i = 0; i < nitems; ++i) {
         const gls_nfft_item& c = items[i];
-        if (c.n_obs < 0 || c.nf < 1 || c.obs_offset < 0 || c.out_offset < 0 ||
-            !std::isfinite(c.f0) || !std::isfinite(c.df) || !(c.df > 0.0)) {
+        if (check_curve(c.obs_offset, c.n_obs, c.nf, c.f0, c.df, c.out_offset)) {
             destroy_plan(P);
             return fail(GLS_E_INVALID, "invalid NFFT item " + std::to_string(i));
         }
@@ -424,10 +474,20 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
         pts += c.n_obs;
         P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
     }
-    Layout L;
+    std::vector<int32_t> bcurves, tiles;
+    Layout probe, L;
+    rc = plan_nfft_geometry(P, 1, probe, pts, bcurves);
+    if (rc != GLS_OK) {
+        destroy_plan(P);
+        return rc;
+    }
+    reserve_meta(P, L, bcurves.size(), 0);
     P->off_state = L.take(sizeof(CurveState) * (size_t)nitems);
-    rc = plan_nfft_geometry(P, 1, L, pts);
-    if (rc == GLS_OK) rc = finish_plan(P, L);
+    const size_t shift = L.cur;
+    for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_bkt, &P->off_grid}) *o += shift;
+    L.cur += probe.cur;
+    fill_meta(P, bcurves, tiles);
+    rc = finish_plan(P, L);
     if (rc != GLS_OK) {
         destroy_plan(P);
         return rc;
@@ -491,31 +551,7 @@ int gls_nfft_execute(gls_plan* plan, const double* t, const void* y, int32_t y_i
     if (plan->kind != 1) return fail(GLS_E_INVALID, "plan is not an NFFT plan");
     if (plan->ncurves == 0) return GLS_OK;
     if (!t || !y || !out || !workspace) return fail(GLS_E_INVALID, "NULL device pointer");
-    hipStream_t s = (hipStream_t)stream;
-    char* base = (char*)workspace;
-    CurveState* state = (CurveState*)(base + plan->off_state);
-    PointArrays pa = point_arrays(plan, workspace);
-    int32_t* bkt = (int32_t*)(base + plan->off_bkt);
-    float2* grids = (float2*)(base + plan->off_grid);
-    int ev = 0;
-    auto mark = [&]() -> hipError_t {
-        if (!plan->timing) return hipSuccess;
-        return hipEventRecord(plan->events[ev++], s);
-    };
-    GLS_HIP_TRY(mark());
-    GLS_HIP_TRY(launch_prep(plan->d_geom, plan->ncurves, state, t, y, y, pa, bkt, plan->pp,
-                            y_is_f32 ? 1 : 0, s));
-    GLS_HIP_TRY(mark());
-    for (const auto& B : plan->buckets) {
-        const int32_t* curves = plan->d_bucket_curves + B.first;
-        GLS_HIP_TRY(launch_pass1(B.n1, plan->d_geom, curves, B.count, 1, B.p2, pa, bkt, grids,
-                                 plan->pp.m, s));
-        GLS_HIP_TRY(mark());
-        GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, plan->d_geom, state, curves, B.count, grids,
-                                      (float2*)out, s));
-        GLS_HIP_TRY(mark());
-    }
-    return GLS_OK;
+    return run_plan(plan, t, y, y, out, 0, y_is_f32 ? 1 : 0, workspace, (hipStream_t)stream);
 }
 
 int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs, int32_t ng, int32_t m,
@@ -564,13 +600,16 @@ int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* grid, flo
     return GLS_OK;
 }
 
-int gls_topk(const gls_plan* plan, const float* power, const uint8_t* mask, int32_t k,
-             float* top_power, int32_t* top_index, void* stream) {
+int gls_topk(const gls_plan* plan, const void* workspace, const float* power,
+             const uint8_t* mask, int32_t k, float* top_power, int32_t* top_index,
+             void* stream) {
     int rc = require_init();
     if (rc) return rc;
-    if (!plan || !power || !top_power || !top_index) return fail(GLS_E_INVALID, "NULL argument");
+    if (!plan || !workspace || !power || !top_power || !top_index)
+        return fail(GLS_E_INVALID, "NULL argument");
     if (k < 1 || k > 64) return fail(GLS_E_INVALID, "k must be in [1, 64]");
-    GLS_HIP_TRY(launch_topk(plan->d_geom, plan->ncurves, power, mask, k, top_power, top_index,
+    const CurveGeom* d_geom = (const CurveGeom*)((const char*)workspace + plan->off_geom);
+    GLS_HIP_TRY(launch_topk(d_geom, plan->ncurves, power, mask, k, top_power, top_index,
                             (hipStream_t)stream));
     return GLS_OK;
 }

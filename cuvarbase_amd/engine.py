@@ -225,7 +225,7 @@ class LombBatch:
             dmask = DeviceArray.to_device(np.ascontiguousarray(mask, dtype=np.uint8))
         if self.use_double:
             raise NotImplementedError("topk on f64 powers")
-        _lib.call("gls_topk", self.plan.handle, self.d_power.ptr,
+        _lib.call("gls_topk", self.plan.handle, self.d_ws.ptr, self.d_power.ptr,
                   dmask.ptr if dmask is not None else None, C.c_int32(k), tp.ptr, ti.ptr,
                   _handle(stream))
         p = tp.get()[: self.ncurves * k].reshape(self.ncurves, k)

@@ -140,6 +140,9 @@ int gls_mem_info(size_t* free_bytes, size_t* total_bytes);
  * `curves` is read during the call only. */
 int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves,
                          int32_t ncurves, const gls_params* params);
+/* Plans are host objects (no device memory): their descriptors are copied
+ * into the first bytes of the workspace by every execute.  Destroying a plan
+ * after its execute calls have returned is safe. */
 int gls_plan_destroy(gls_plan* plan);
 /* Device workspace the plan needs (caller allocates, e.g. gls_malloc). */
 int gls_plan_workspace_bytes(const gls_plan* plan, size_t* bytes);
@@ -203,11 +206,12 @@ int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs,
                       double tmax, double spp, float* grid, void* stream);
 
 /* ---- peak extraction (next row: best frequencies, lombscargle.py:1099-1107) */
-/* For each curve of a finished plan: the top-k powers and their indices,
- * ignoring indices where mask[out_offset + k] != 0 (mask may be NULL).
- * top_power: ncurves*k f32, top_index: ncurves*k i32 (descending power,
- * ties by lower index; -1 where fewer than k valid values). */
-int gls_topk(const gls_plan* plan, const float* power,
+/* For each curve of a plan executed into `power` with `workspace`: the top-k
+ * powers and their indices, ignoring indices where mask[out_offset + k] != 0
+ * (mask may be NULL).  top_power: ncurves*k f32, top_index: ncurves*k i32
+ * (descending power, ties by lower index; -1 where fewer than k valid
+ * values).  Enqueue on the stream of the execute (or after it). */
+int gls_topk(const gls_plan* plan, const void* workspace, const float* power,
              const uint8_t* mask, int32_t k, float* top_power,
              int32_t* top_index, void* stream);
 

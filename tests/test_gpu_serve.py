@@ -1,0 +1,62 @@
+"""Continuous-batching server on the GPU."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(n=40, rate=2000.0, seed=41):
+    from cuvarbase_amd import synthetic
+
+    return synthetic.stream_requests(rate, nreq=n, seed=seed)
+
+
+def test_each_result_same_alone_and_in_the_mix():
+    from cuvarbase_amd import engine
+    from cuvarbase_amd.serve import GLSServer
+
+    arrivals, data, freqs = _stream()
+    srv = GLSServer(hbm_budget=8e9, slots=3, max_batch_evals=2e7)
+    st = srv.serve_stream(arrivals, data, freqs)
+    assert st["batches"] > 1
+    worst = 0.0
+    for r, d, f in zip(st["requests_obj"], data, freqs):
+        assert r.done and r.latency > 0
+        alone = engine.LombBatch([d], [f], sigma=srv.sigma, m=srv.m).run()[0]
+        worst = max(worst, float(np.max(np.abs(r.power.astype(float) - alone))))
+    print(f"served {st['requests']} requests in {st['batches']} batches, p50 {st['p50_ms']:.1f} "
+          f"ms, p99 {st['p99_ms']:.1f} ms; alone vs mix max|dP| = {worst:.2e}")
+    assert worst < 1e-6
+
+
+def test_admission_waits_for_memory():
+    from cuvarbase_amd.engine import LombBatch
+    from cuvarbase_amd.serve import GLSServer
+
+    arrivals, data, freqs = _stream(n=12, seed=42)
+    # budget for about two of the largest requests at a time
+    sizes = []
+    for d, f in zip(data, freqs):
+        b = LombBatch([d], [f], sigma=1.2, m=18)
+        sizes.append(b.plan.workspace_bytes + 24 * len(d[0]) + 4 * len(f) + 3 * 256)
+    srv = GLSServer(hbm_budget=2.2 * max(sizes), slots=4, max_batch_evals=1e9, sigma=1.2, m=18)
+    for (t, y, dy), f in zip(data, freqs):
+        srv.submit(t, y, dy, f)
+    peak = 0
+    while srv.busy:
+        srv.poll()
+        peak = max(peak, srv.arena.used)
+    assert peak <= srv.arena.nbytes
+    assert len(srv.completed) == len(data)
+    assert srv.batches_launched >= 2
+
+
+def test_request_larger_than_budget_raises():
+    from cuvarbase_amd.serve import GLSServer
+
+    _, data, freqs = _stream(n=1, seed=43)
+    srv = GLSServer(hbm_budget=1 << 20, slots=1)
+    t, y, dy = data[0]
+    srv.submit(t, y, dy, freqs[0])
+    with pytest.raises(MemoryError):
+        srv.drain()
