@@ -30,6 +30,7 @@ INPUT_WEIGHTED = 1
 
 METHOD_NFFT = 0
 METHOD_DIRECT = 1
+METHOD_DIRECT_CUSTOM = 2
 
 
 class GLSError(RuntimeError):
@@ -102,6 +103,7 @@ _PROTOS = {
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "gls_lomb_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "gls_lomb_execute_custom": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp],
     "gls_nfft_plan_create": [C.POINTER(_vp), C.POINTER(gls_nfft_item), C.c_int32,
                              C.c_double, C.c_int32],
     "gls_nfft_execute": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],

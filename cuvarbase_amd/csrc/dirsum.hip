@@ -95,6 +95,60 @@ __global__ __launch_bounds__(NT) void dirsum_kernel(const CurveGeom* __restrict_
     }
 }
 
+// Direct sums at explicit frequencies freqs[out + k] (any spacing): the work of
+// lomb_dirsum_custom_frq (cuvarbase/kernels/lomb.cu:167-187).  One frequency
+// per thread, each phase reduced mod 1 in FP64.
+template <int OUT>
+__global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __restrict__ geom,
+                                                           const CurveState* __restrict__ state,
+                                                           const int32_t* __restrict__ tile_curve,
+                                                           const int32_t* __restrict__ tile_first,
+                                                           PointArrays pa, PlanParams pp,
+                                                           const double* __restrict__ freqs,
+                                                           void* __restrict__ out) {
+    __shared__ double sdt[NT];
+    __shared__ float sw[NT];
+    __shared__ float syw[NT];
+    const int tid = threadIdx.x;
+    const int ci = tile_curve[blockIdx.x];
+    const CurveGeom& G = geom[ci];
+    const int N = G.n_obs;
+    const int k = tile_first[blockIdx.x] + tid;
+    const double f = (k < G.nf) ? freqs[G.out + k] : 0.0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base < N; base += NT) {
+        const int j = base + tid;
+        if (j < N) {
+            sdt[tid] = pa.dt[G.pts + j];
+            sw[tid] = pa.w[G.pts + j];
+            syw[tid] = pa.yw[G.pts + j];
+        }
+        __syncthreads();
+        const int cnt = min(NT, N - base);
+        float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int jj = 0; jj < cnt; ++jj) {
+            const float2 z = expi_rev(f * sdt[jj]);
+            const float w = sw[jj], yw = syw[jj];
+            a[0] += w * z.x;
+            a[1] += w * z.y;
+            a[2] += w * (z.x * z.x - z.y * z.y);
+            a[3] += w * (2.f * z.x * z.y);
+            a[4] += yw * z.x;
+            a[5] += yw * z.y;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += (double)a[q];
+        __syncthreads();
+    }
+    if (k < G.nf) {
+        const CurveState st = state[ci];
+        const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
+                                   st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+        if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+        else ((double*)out)[G.out + k] = P;
+    }
+}
+
 template <int RF>
 hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
                            const int32_t* tile_curve, const int32_t* tile_first, int ntiles,
@@ -110,6 +164,20 @@ hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
 }
 
 }  // namespace
+
+hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,
+                                const int32_t* tile_curve, const int32_t* tile_first,
+                                int ntiles, PointArrays pa, const PlanParams& pp,
+                                const double* freqs, void* power, int out_f64, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (out_f64)
+        hipLaunchKernelGGL(dirsum_custom_kernel<1>, dim3(ntiles), dim3(NT), 0, s, d_geom, state,
+                           tile_curve, tile_first, pa, pp, freqs, power);
+    else
+        hipLaunchKernelGGL(dirsum_custom_kernel<0>, dim3(ntiles), dim3(NT), 0, s, d_geom, state,
+                           tile_curve, tile_first, pa, pp, freqs, power);
+    return hipGetLastError();
+}
 
 hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const int32_t* tile_curve, const int32_t* tile_first, int ntiles,

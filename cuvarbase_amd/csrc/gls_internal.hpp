@@ -119,6 +119,11 @@ hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          int ntiles, int rf, PointArrays pa,
                          const PlanParams& pp, void* power, int out_f64,
                          hipStream_t s);
+hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,
+                                const int32_t* tile_curve, const int32_t* tile_first,
+                                int ntiles, PointArrays pa, const PlanParams& pp,
+                                const double* freqs, void* power, int out_f64,
+                                hipStream_t s);
 hipError_t launch_gaussian_grid(const double* t, const double* y, int n_obs,
                                 int ng, int m, double b, double tmin,
                                 double tmax, double spp, float* grid,

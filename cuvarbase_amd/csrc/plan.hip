@@ -171,7 +171,7 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
 }
 
 int plan_direct_tiles(gls_plan* P, std::vector<int32_t>& tiles) {
-    int rf = 8;
+    int rf = (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) ? 1 : 8;
     while (rf > 1) {
         int64_t nt = 0;
         for (auto& G : P->geom) nt += (G.nf + 256 * rf - 1) / (256 * rf);
@@ -269,7 +269,7 @@ PointArrays point_arrays(const gls_plan* P, void* ws) {
 
 // The whole launch sequence of a plan on stream s.
 int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* out,
-             int out_f64, int y_is_f32, void* ws, hipStream_t s) {
+             int out_f64, int y_is_f32, void* ws, hipStream_t s, const double* freqs = nullptr) {
     char* base = (char*)ws;
     if (!P->meta.empty())
         GLS_HIP_TRY(hipMemcpyAsync(base, P->meta.data(), P->meta.size(), hipMemcpyHostToDevice,
@@ -304,6 +304,10 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
                                               P->pp, out, out_f64, s));
             GLS_HIP_TRY(mark());
         }
+    } else if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) {
+        GLS_HIP_TRY(launch_dirsum_custom(d_geom, state, d_tiles, d_tiles + P->ntiles, P->ntiles,
+                                         pa, P->pp, freqs, out, out_f64, s));
+        GLS_HIP_TRY(mark());
     } else {
         GLS_HIP_TRY(launch_dirsum(d_geom, state, d_tiles, d_tiles + P->ntiles, P->ntiles, P->rf,
                                   pa, P->pp, out, out_f64, s));
@@ -320,6 +324,8 @@ int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, voi
     if (P->kind != 0) return fail(GLS_E_INVALID, "plan is not a Lomb-Scargle plan");
     if (P->ncurves == 0) return GLS_OK;
     if (!t || !a || !b || !power || !ws) return fail(GLS_E_INVALID, "NULL device pointer");
+    if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM)
+        return fail(GLS_E_INVALID, "custom-frequency plans run with gls_lomb_execute_custom");
     return run_plan(P, t, a, b, power, out_f64, 0, ws, (hipStream_t)stream);
 }
 
@@ -354,7 +360,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     if (prm.mode < 0 || prm.mode > 2) return fail(GLS_E_INVALID, "mode must be 0, 1 or 2");
     if (prm.input_kind != GLS_INPUT_RAW && prm.input_kind != GLS_INPUT_WEIGHTED)
         return fail(GLS_E_INVALID, "unknown input_kind");
-    if (prm.method != GLS_METHOD_NFFT && prm.method != GLS_METHOD_DIRECT)
+    if (prm.method != GLS_METHOD_NFFT && prm.method != GLS_METHOD_DIRECT &&
+        prm.method != GLS_METHOD_DIRECT_CUSTOM)
         return fail(GLS_E_INVALID, "unknown method");
     if (prm.method == GLS_METHOD_NFFT) {
         if (!(prm.sigma >= 1.0 && prm.sigma <= 64.0))
@@ -378,7 +385,9 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     int64_t pts = 0;
     for (int i = 0; i < ncurves; ++i) {
         const gls_curve& c = curves[i];
-        if (check_curve(c.obs_offset, c.n_obs, c.nf, c.f0, c.df, c.out_offset)) {
+        const bool custom = prm.method == GLS_METHOD_DIRECT_CUSTOM;
+        if (check_curve(c.obs_offset, c.n_obs, c.nf, custom ? 0.0 : c.f0, custom ? 1.0 : c.df,
+                        c.out_offset)) {
             destroy_plan(P);
             return fail(GLS_E_INVALID, "invalid curve descriptor " + std::to_string(i));
         }
@@ -397,7 +406,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
     }
     std::vector<int32_t> bcurves, tiles;
-    if (prm.method == GLS_METHOD_DIRECT) rc = plan_direct_tiles(P, tiles);
+    if (prm.method != GLS_METHOD_NFFT) rc = plan_direct_tiles(P, tiles);
     if (rc != GLS_OK) {
         destroy_plan(P);
         return rc;
@@ -536,6 +545,21 @@ int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n, int32_t
 int gls_lomb_execute(gls_plan* plan, const double* t, const void* a, const void* b, float* power,
                      void* workspace, void* stream) {
     return execute_lomb(plan, t, a, b, power, 0, workspace, stream);
+}
+
+int gls_lomb_execute_custom(gls_plan* plan, const double* t, const void* a, const void* b,
+                            const double* freqs, void* power, int32_t power_is_f64,
+                            void* workspace, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (plan->kind != 0 || plan->pp.method != GLS_METHOD_DIRECT_CUSTOM)
+        return fail(GLS_E_INVALID, "plan was not created with GLS_METHOD_DIRECT_CUSTOM");
+    if (plan->ncurves == 0) return GLS_OK;
+    if (!t || !a || !b || !freqs || !power || !workspace)
+        return fail(GLS_E_INVALID, "NULL device pointer");
+    return run_plan(plan, t, a, b, power, power_is_f64 ? 1 : 0, 0, workspace,
+                    (hipStream_t)stream, freqs);
 }
 
 int gls_lomb_execute_f64(gls_plan* plan, const double* t, const void* a, const void* b,

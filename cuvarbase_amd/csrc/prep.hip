@@ -120,7 +120,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
         }
     };
 
-    if (method == GLS_METHOD_DIRECT) {
+    if (method == GLS_METHOD_DIRECT || method == GLS_METHOD_DIRECT_CUSTOM) {
         for (int j = tid; j < N; j += NT) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             double w, yw;

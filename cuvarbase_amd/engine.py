@@ -121,12 +121,14 @@ class LombBatch:
     ----------
     data : list of (t, y, dy) (``input_kind='raw'``) or (t, yw, w) with
         per-curve ``ybar``/``yy`` (``input_kind='weighted'``)
-    freqs : list of linear frequency grids (one per curve)
+    freqs : list of linear frequency grids (one per curve); with
+        ``custom_freqs=True`` any 1-D arrays of frequencies, evaluated by
+        direct sums at exactly those values
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
-                 input_kind="raw", ybar=None, yy=None):
+                 input_kind="raw", ybar=None, yy=None, custom_freqs=False):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)
@@ -138,8 +140,15 @@ class LombBatch:
         self.n_obs = np.empty(self.ncurves, dtype=np.int64)
         self.nf = np.empty(self.ncurves, dtype=np.int64)
         self.out_offsets = np.empty(self.ncurves, dtype=np.int64)
+        self.custom = bool(custom_freqs)
         for i, (d, fr) in enumerate(zip(data, freqs)):
-            f0, df, nf = linear_grid(fr)
+            if self.custom:
+                fr = np.asarray(fr, dtype=np.float64)
+                if fr.ndim != 1 or len(fr) < 1 or not np.all(np.isfinite(fr)):
+                    raise ValueError("custom frequencies must be a finite 1-D array")
+                f0, df, nf = 0.0, 1.0, len(fr)
+            else:
+                f0, df, nf = linear_grid(fr)
             n = len(d[0])
             c = _lib.gls_curve(obs_offset=obs, n_obs=n, nf=nf, f0=f0, df=df, out_offset=out,
                                ybar=0.0 if ybar is None else float(ybar[i]),
@@ -152,8 +161,13 @@ class LombBatch:
             out += nf
         self.total_obs = obs
         self.total_out = out
-        self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind,
-                              _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT)
+        if self.custom:
+            method = _lib.METHOD_DIRECT_CUSTOM
+            self._freqs = np.concatenate([np.asarray(f, dtype=np.float64) for f in freqs])
+        else:
+            method = _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT
+            self._freqs = None
+        self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method)
         # host staging of the inputs
         t = np.empty(obs, dtype=np.float64)
         if self.input_kind == _lib.INPUT_RAW:
@@ -170,7 +184,7 @@ class LombBatch:
             b[o:o + n] = d[2]
             o += n
         self._host = (t, a, b)
-        self.d_t = self.d_a = self.d_b = self.d_power = self.d_ws = None
+        self.d_t = self.d_a = self.d_b = self.d_power = self.d_ws = self.d_freqs = None
 
     # ---- device side ---------------------------------------------------
     def allocate(self):
@@ -180,6 +194,8 @@ class LombBatch:
         self.d_b = DeviceArray(max(len(b), 1), b.dtype)
         self.d_power = DeviceArray(max(self.total_out, 1), self.out_dtype)
         self.d_ws = DeviceArray(self.plan.workspace_bytes, np.uint8)
+        if self.custom:
+            self.d_freqs = DeviceArray(max(self.total_out, 1), np.float64)
         return self
 
     def upload(self, stream=None):
@@ -190,9 +206,16 @@ class LombBatch:
             self.d_t.set_async(t, stream=stream)
             self.d_a.set_async(a, stream=stream)
             self.d_b.set_async(b, stream=stream)
+        if self.custom:
+            self.d_freqs.set_async(self._freqs, stream=stream)
         return self
 
     def execute(self, stream=None):
+        if self.custom:
+            _lib.call("gls_lomb_execute_custom", self.plan.handle, self.d_t.ptr, self.d_a.ptr,
+                      self.d_b.ptr, self.d_freqs.ptr, self.d_power.ptr,
+                      C.c_int32(1 if self.use_double else 0), self.d_ws.ptr, _handle(stream))
+            return self
         fn = "gls_lomb_execute_f64" if self.use_double else "gls_lomb_execute"
         _lib.call(fn, self.plan.handle, self.d_t.ptr, self.d_a.ptr, self.d_b.ptr,
                   self.d_power.ptr, self.d_ws.ptr, _handle(stream))

@@ -486,11 +486,26 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
             check_k0(frq, k0=k0)
         return frqs, k0s
 
-    def run(self, data, use_fft=True, memory=None, freqs=None, **kwargs):
+    def run(self, data, use_fft=True, memory=None, freqs=None, custom_freqs=False,
+            **kwargs):
         """GLS periodograms of [(t, y, dy), ...]; returns [(freqs_i, P_i)]
-        with P_i valid after finish() (cuvarbase/lombscargle.py:923-1010)."""
+        with P_i valid after finish() (cuvarbase/lombscargle.py:923-1010).
+
+        ``custom_freqs=True`` evaluates exactly the given (arbitrarily
+        spaced) frequencies by direct sums, the job of the reference's
+        ``lomb_dirsum_custom_frq`` (cuvarbase/kernels/lomb.cu:167-187), which
+        its Python layer never launches; the reference requires a linear grid
+        (cuvarbase/lombscargle.py:939-940)."""
         self._ready(**kwargs)
         data = normalize_light_curves(data)
+        if custom_freqs:
+            if freqs is None:
+                raise ValueError("custom_freqs=True needs explicit frequencies")
+            frqs = [np.asarray(freqs, dtype=np.float64)] * len(data) \
+                if np.isscalar(freqs[0]) else [np.asarray(f, np.float64) for f in freqs]
+            if len(frqs) != len(data):
+                raise ValueError("need one frequency array per light curve")
+            return self._run_batch(data, frqs, use_fft=False, custom=True, **kwargs)
         frqs, k0s = self._frequencies(data, freqs, **kwargs)
         if memory is None:
             memory = self.memory
@@ -507,7 +522,7 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         return [(f, r) for f, r in zip(frqs, results)]
 
     def _run_batch(self, data, frqs, use_fft=True, python_dir_sums=False,
-                   transfer_to_host=True, **kwargs):
+                   transfer_to_host=True, custom=False, **kwargs):
         if python_dir_sums:
             out = []
             for (t, y, dy), f in zip(data, frqs):
@@ -521,7 +536,8 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         m = max(self.nfft_proc.get_m(len(f)) for f in frqs) if frqs else self.nfft_proc.m
         batch = LombBatch(data, frqs, sigma=self.nfft_proc.sigma, m=m, mode=mode,
                           amplitude_prior=kwargs.get('amplitude_prior', None),
-                          use_fft=use_fft, use_double=self.use_double, input_kind="raw")
+                          use_fft=use_fft, use_double=self.use_double, input_kind="raw",
+                          custom_freqs=custom)
         if not self.streams:
             self._create_streams(1)
         s = self.streams[0]

@@ -53,6 +53,10 @@ extern "C" {
 /* ---- methods ------------------------------------------------------------ */
 #define GLS_METHOD_NFFT     0   /* use_fft=True  (cuvarbase/lombscargle.py:660-684) */
 #define GLS_METHOD_DIRECT   1   /* use_fft=False (cuvarbase/lombscargle.py:643-659) */
+/* direct sums at explicit, arbitrarily spaced frequencies: the reference's
+ * lomb_dirsum_custom_frq kernel (cuvarbase/kernels/lomb.cu:167-187), which
+ * its Python layer never launches; run with gls_lomb_execute_custom */
+#define GLS_METHOD_DIRECT_CUSTOM 2
 
 /* One light curve of a batch.  Observations of curve i are
  * [obs_offset, obs_offset + n_obs) of the input arrays (any order); its nf
@@ -167,6 +171,14 @@ int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n,
 int gls_lomb_execute(gls_plan* plan, const double* t, const void* a,
                      const void* b, float* power, void* workspace,
                      void* stream);
+
+/* Custom-frequency plans (GLS_METHOD_DIRECT_CUSTOM): curve i is evaluated
+ * at freqs[out_offset .. out_offset + nf) (device f64; f0/df ignored) into
+ * f32 (power_is_f64 = 0) or f64 powers.  Replaces lomb_dirsum_custom_frq
+ * (cuvarbase/kernels/lomb.cu:167-187). */
+int gls_lomb_execute_custom(gls_plan* plan, const double* t, const void* a,
+                            const void* b, const double* freqs, void* power,
+                            int32_t power_is_f64, void* workspace, void* stream);
 
 /* Same, into an f64 power array (use_double=True,
  * cuvarbase/lombscargle.py:96-101). */

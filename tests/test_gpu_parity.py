@@ -305,3 +305,27 @@ def test_topk_matches_host():
     mask[b.out_offsets[0] + ti[0, 0]] = 1
     tp2, ti2 = b.topk(2, mask=mask)
     assert ti2[0, 0] == ti[0, 1]
+
+
+@pytest.mark.parametrize("use_double", [False, True])
+def test_custom_frequencies_against_oracle(use_double):
+    """Arbitrarily spaced frequencies (log grid, duplicates, unsorted) by
+    direct sums, against FP64 direct sums at the same frequencies."""
+    data = [_curve(71, n=500), _curve(72, n=37, T=3.0)]
+    rng = np.random.default_rng(7)
+    f_log = np.geomspace(0.01, 20.0, 3001)
+    f_odd = np.concatenate([rng.uniform(0.05, 9.0, 700), [0.5882352941, 0.5882352941]])
+    for mode_kw in ({}, {"floating_mean": False}):
+        P = _run(data, [f_log, f_odd], use_fft=False, custom_freqs=True,
+                 use_double=use_double, **mode_kw)
+        mode = og.FLOATING_MEAN if mode_kw == {} else og.STANDARD
+        for (t, y, dy), f, p in zip(data, [f_log, f_odd], P):
+            ref = og.gls_direct(t, y, dy, f, mode=mode)
+            assert p.shape == f.shape
+            assert np.max(np.abs(p - ref)) < 1e-5
+            assert np.argmax(p) == np.argmax(ref)
+    # a linear grid through the custom path equals the linear direct path
+    f_lin = 0.01 * (3 + np.arange(4000))
+    a = _run(data[:1], f_lin, use_fft=False, custom_freqs=True)[0]
+    b = _run(data[:1], f_lin, use_fft=False)[0]
+    assert np.max(np.abs(a - b)) < 1e-6
