@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--stream-requests", type=int, default=1500,
                     help="requests per arrival rate in the serving leg (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one plain elementwise copy of this many MiB before the "
+                         "timed steps (scales HBM counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the GLS path "
@@ -116,6 +119,14 @@ def main():
     for _ in range(args.warmup):
         batch.execute(stream=s)
     s.synchronize()
+    if args.calib_copy_mib > 0:
+        import torch
+
+        x = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
 
     stage_acc = {}
     e0, e1 = _device.Event(), _device.Event()

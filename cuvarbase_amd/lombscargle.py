@@ -532,6 +532,13 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
                 yy = np.dot(w, (y - ybar) ** 2)
                 out.append((f, lomb_scargle_direct_sums(t, yw, w, f, yy)))
             return out
+        batch = self._launch_batch(data, frqs, use_fft=use_fft, custom=custom, **kwargs)
+        flat = batch.download_async(stream=self.streams[0]) if transfer_to_host else \
+            pinned_zeros(max(batch.total_out, 1), batch.out_dtype)
+        return [(f, p) for f, p in zip(frqs, batch.split(flat))]
+
+    def _launch_batch(self, data, frqs, use_fft=True, custom=False, **kwargs):
+        """Plan, upload and execute one LombBatch on streams[0] (no download)."""
         mode = _mode(kwargs.get('floating_mean', True), kwargs.get('window', False))
         m = max(self.nfft_proc.get_m(len(f)) for f in frqs) if frqs else self.nfft_proc.m
         batch = LombBatch(data, frqs, sigma=self.nfft_proc.sigma, m=m, mode=mode,
@@ -543,10 +550,8 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         s = self.streams[0]
         batch.upload(stream=s)
         batch.execute(stream=s)
-        flat = batch.download_async(stream=s) if transfer_to_host else \
-            pinned_zeros(max(batch.total_out, 1), batch.out_dtype)
         self._inflight.append(batch)
-        return [(f, p) for f, p in zip(frqs, batch.split(flat))]
+        return batch
 
     def batched_run_const_nfreq(self, data, batch_size=10, use_fft=True, freqs=None,
                                 only_return_best_freqs=False, ignore_freq_mask=None,
@@ -566,8 +571,29 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         mask = np.ones(len(freqs), dtype=bool) if ignore_freq_mask is None else \
             ~np.asarray(ignore_freq_mask)
         lsps, best_freqs, best_sig = [], [], []
+        device_peaks = only_return_best_freqs and not self.use_double and \
+            not kwargs.get('python_dir_sums', False) and self.memory is None
         for start in range(0, len(data), batch_size):
             batch = data[start:start + batch_size]
+            if device_peaks:
+                # best frequency on the device (gls_topk, k = 1, masked); only
+                # one value per curve comes back.  The reference evaluates
+                # fap_baluev on the whole masked array and reads it at the
+                # argmax (:1099-1107): the same number as evaluating it there.
+                lb = self._launch_batch(batch, [freqs] * len(batch), use_fft=use_fft,
+                                        **kwargs)
+                skip = None if ignore_freq_mask is None else \
+                    np.tile(~mask, len(batch)).astype(np.uint8)
+                top_p, top_i = lb.topk(1, mask=skip, stream=self.streams[0])
+                self.finish()
+                for i in range(len(batch)):
+                    bi = int(top_i[i, 0])
+                    z = float(top_p[i, 0]) if bi >= 0 else np.nan
+                    fap = fap_baluev(batch[i][0], batch[i][2], np.array([z]),
+                                     np.max(freqs[mask]))
+                    best_freqs.append(freqs[bi] if bi >= 0 else np.nan)
+                    best_sig.append(1. - fap[0])
+                continue
             results = self.run(batch, freqs=freqs, use_fft=use_fft, **kwargs)
             self.finish()
             for i, (f, p) in enumerate(results):

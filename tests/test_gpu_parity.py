@@ -288,6 +288,19 @@ def test_drop_in_paths_agree():
     for (f, p), bf in zip(res, best):
         assert bf == f[np.argmax(p)]
     assert all(0 <= s <= 1 for s in sig)
+    # significance of the device-selected peak = host FAP at the argmax
+    from cuvarbase_amd.lombscargle import fap_baluev
+    for d, (f, p), sg in zip(datas, res, sig):
+        z = np.array([p[np.argmax(p)]], dtype=np.float64)
+        assert abs(sg - (1.0 - fap_baluev(d[0], d[2], z, f.max())[0])) < 1e-6
+    # ignore_freq_mask removes the best peak of curve 0
+    ignore = np.zeros(len(fshared), dtype=bool)
+    ignore[np.argmax(res[0][1])] = True
+    best2, _ = proc.batched_run_const_nfreq(datas, batch_size=3, samples_per_peak=3,
+                                            only_return_best_freqs=True,
+                                            ignore_freq_mask=ignore)
+    p0 = np.where(ignore, -np.inf, res[0][1])
+    assert best2[0] == fshared[np.argmax(p0)]
 
 
 def test_topk_matches_host():
