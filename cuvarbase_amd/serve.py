@@ -1,12 +1,14 @@
 """Continuous-batching GLS server on one GPU.
 
-Requests (one light curve + a linear frequency grid each) are queued as they
-arrive.  Whenever one of ``slots`` batch slots is free, the server coalesces
-the queued requests (FIFO, up to ``max_batch_evals`` curve x frequency
-evaluations) into one planned batch, sub-allocates its device memory from a
+Requests (one light curve + a linear frequency grid, or an explicit list of
+frequencies, each) are queued as they arrive.  Whenever one of ``slots`` batch
+slots is free, the server coalesces the queued requests of the oldest
+request's class (same method, GLS mode and result kind; FIFO, up to
+``max_batch_evals`` curve x frequency evaluations) into one planned batch, sub-allocates its device memory from a
 resident arena (admission by HBM budget: a batch that does not fit waits for
 memory to be released), and launches it on the slot's stream: H2D of the raw
-(t, y, dy) -> prep -> passes -> D2H into the slot's pinned buffer.  Batches of
+(t, y, dy) -> prep -> passes [-> top-k] -> D2H into the slot's pinned buffer
+(the whole periodogram, or only k peaks per curve for top-k requests).  Batches of
 different slots overlap on the device; requests that arrive meanwhile join the
 next batch.  ``poll`` retires finished batches (event query, no blocking) and
 launches new ones.
@@ -30,6 +32,7 @@ from . import _lib
 from ._device import DeviceArray, Event, Stream, pinned_empty, _handle
 from .engine import Plan, linear_grid
 from .tuning import choose_nfft_params
+from . import wire
 
 
 class FreeList:
@@ -84,19 +87,37 @@ class DeviceArena(FreeList):
 
 
 class Request:
-    """One light curve submitted to the server."""
+    """One light curve submitted to the server.
+
+    ``freqs`` is a linear grid (evaluated by the NFFT path) unless
+    ``custom_freqs`` is set (direct sums at exactly those frequencies).
+    ``k > 0`` asks for the k highest peaks (power, index, frequency and
+    Baluev false-alarm probability) instead of the whole periodogram."""
 
     _ids = itertools.count()
 
-    def __init__(self, t, y, dy, freqs, arrival=None):
-        self.id = next(self._ids)
+    def __init__(self, t, y, dy, freqs, arrival=None, custom_freqs=False, k=0,
+                 mode=None, rid=None):
+        self.id = next(self._ids) if rid is None else int(rid)
         self.t = np.ascontiguousarray(t, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
         self.dy = np.ascontiguousarray(dy, dtype=np.float64)
-        self.f0, self.df, self.nf = linear_grid(freqs)
+        if custom_freqs:
+            self.freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+            if self.freqs.ndim != 1 or len(self.freqs) < 1:
+                raise ValueError("custom frequencies must be a non-empty 1-D array")
+            self.f0, self.df, self.nf = 0.0, 1.0, len(self.freqs)
+        else:
+            self.freqs = None
+            self.f0, self.df, self.nf = linear_grid(freqs)
+        self.k = int(k)
+        if self.k < 0 or self.k > 64:
+            raise ValueError("k must be in [0, 64]")
+        self.mode = mode
         self.arrival = time.perf_counter() if arrival is None else float(arrival)
         self.done_at = None
         self.power = None
+        self.top_power = self.top_index = self.top_freq = self.fap = None
 
     @property
     def latency(self):
@@ -104,7 +125,13 @@ class Request:
 
     @property
     def done(self):
-        return self.power is not None
+        return self.done_at is not None
+
+    def frequency(self, idx):
+        idx = np.asarray(idx)
+        if self.freqs is not None:
+            return np.where(idx >= 0, self.freqs[np.maximum(idx, 0)], np.nan)
+        return np.where(idx >= 0, self.f0 + self.df * idx, np.nan)
 
 
 class _Slot:
@@ -113,24 +140,31 @@ class _Slot:
         self.event = Event()
         self.h_in = None      # pinned (3, max_obs) f64
         self.h_out = None     # pinned f32
+        self.h_freq = None    # pinned f64 (custom frequency lists)
+        self.h_idx = None     # pinned i32 (top-k indices)
         self.batch = None
 
-    def staging(self, nobs, nout):
+    def staging(self, nobs, nout, nfreq=0, nidx=0):
         if self.h_in is None or self.h_in.shape[1] < nobs:
             self.h_in = pinned_empty((3, max(nobs, 1024)), np.float64)
         if self.h_out is None or self.h_out.size < nout:
             self.h_out = pinned_empty(max(nout, 1024), np.float32)
+        if nfreq and (self.h_freq is None or self.h_freq.size < nfreq):
+            self.h_freq = pinned_empty(max(nfreq, 1024), np.float64)
+        if nidx and (self.h_idx is None or self.h_idx.size < nidx):
+            self.h_idx = pinned_empty(max(nidx, 1024), np.int32)
         return self.h_in, self.h_out
 
 
 class _Batch:
-    def __init__(self, reqs, plan, blocks, nobs, nout, offsets):
+    def __init__(self, reqs, plan, blocks, nobs, nout, offsets, k=0):
         self.reqs = reqs
         self.plan = plan
         self.blocks = blocks
         self.nobs = nobs
         self.nout = nout
         self.offsets = offsets
+        self.k = k
         self.t_launch = time.perf_counter()
 
 
@@ -162,8 +196,10 @@ class GLSServer:
         self.evals_done = 0
 
     # -- client side ---------------------------------------------------------
-    def submit(self, t, y, dy, freqs, arrival=None):
-        r = Request(t, y, dy, freqs, arrival=arrival)
+    def submit(self, t, y, dy, freqs, arrival=None, custom_freqs=False, k=0, mode=None,
+               rid=None):
+        r = Request(t, y, dy, freqs, arrival=arrival, custom_freqs=custom_freqs, k=k,
+                    mode=mode, rid=rid)
         self.queue.append(r)
         return r
 
@@ -172,8 +208,15 @@ class GLSServer:
         b = slot.batch
         now = time.perf_counter()
         out = slot.h_out
-        for r, o in zip(b.reqs, b.offsets):
-            r.power = np.array(out[o:o + r.nf])
+        for i, (r, o) in enumerate(zip(b.reqs, b.offsets)):
+            if b.k:
+                k = r.k
+                r.top_power = np.array(out[i * b.k:i * b.k + k])
+                r.top_index = np.array(slot.h_idx[i * b.k:i * b.k + k])
+                r.top_freq = r.frequency(r.top_index)
+                r.fap = _fap(r, r.top_power, r.top_index)
+            else:
+                r.power = np.array(out[o:o + r.nf])
             r.done_at = now
             self.completed.append(r)
             self.evals_done += r.nf
@@ -182,17 +225,29 @@ class GLSServer:
         b.plan.destroy()
         slot.batch = None
 
+    def _class(self, r):
+        return (r.freqs is not None, self.mode if r.mode is None else r.mode, r.k > 0)
+
     def _take_batch(self):
-        reqs, ev = [], 0
+        """FIFO head's class; later requests of other classes keep their place."""
+        if not self.queue:
+            return []
+        cls = self._class(self.queue[0])
+        reqs, rest, ev = [], [], 0
         while self.queue and len(reqs) < self.max_batch_curves:
             r = self.queue[0]
+            if self._class(r) != cls:
+                rest.append(self.queue.popleft())
+                continue
             if reqs and ev + r.nf > self.max_batch_evals:
                 break
             reqs.append(self.queue.popleft())
             ev += r.nf
+        self.queue.extendleft(reversed(rest))
         return reqs
 
     def _launch(self, slot, reqs):
+        custom, mode, topk = self._class(reqs[0])
         curves, obs, out, offsets = [], 0, 0, []
         for r in reqs:
             curves.append(_lib.gls_curve(obs_offset=obs, n_obs=len(r.t), nf=r.nf, f0=r.f0,
@@ -200,9 +255,15 @@ class GLSServer:
             offsets.append(out)
             obs += len(r.t)
             out += r.nf
-        plan = Plan.lomb(curves, self.sigma, self.m, self.mode, 0.0, 0.0, _lib.INPUT_RAW,
-                         _lib.METHOD_NFFT)
+        plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
+                         _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT)
+        k = max(r.k for r in reqs) if topk else 0
+        nreq = len(reqs)
         sizes = [24 * max(obs, 1), 4 * max(out, 1), plan.workspace_bytes]
+        if custom:
+            sizes.append(8 * max(out, 1))
+        if k:
+            sizes.append(8 * nreq * k)
         blocks = []
         for sz in sizes:
             blk = self.arena.alloc(sz)
@@ -212,7 +273,7 @@ class GLSServer:
                 plan.destroy()
                 return False
             blocks.append(blk)
-        h_in, h_out = slot.staging(obs, out)
+        h_in, h_out = slot.staging(obs, nreq * k if k else out, out if custom else 0, nreq * k)
         o = 0
         for r in reqs:
             n = len(r.t)
@@ -222,16 +283,35 @@ class GLSServer:
             o += n
         s = slot.stream.handle
         p_in = self.arena.ptr(blocks[0])
+        p_pow = self.arena.ptr(blocks[1])
+        p_ws = self.arena.ptr(blocks[2])
         row = 8 * max(obs, 1)
-        for k in range(3):
-            _lib.call("gls_memcpy_h2d_async", p_in + k * row, h_in[k].ctypes.data,
+        for q in range(3):
+            _lib.call("gls_memcpy_h2d_async", p_in + q * row, h_in[q].ctypes.data,
                       C.c_size_t(8 * obs), s)
-        _lib.call("gls_lomb_execute", plan.handle, p_in, p_in + row, p_in + 2 * row,
-                  self.arena.ptr(blocks[1]), self.arena.ptr(blocks[2]), s)
-        _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, self.arena.ptr(blocks[1]),
-                  C.c_size_t(4 * out), s)
+        if custom:
+            for r, o in zip(reqs, offsets):
+                slot.h_freq[o:o + r.nf] = r.freqs
+            p_fr = self.arena.ptr(blocks[3])
+            _lib.call("gls_memcpy_h2d_async", p_fr, slot.h_freq.ctypes.data,
+                      C.c_size_t(8 * out), s)
+            _lib.call("gls_lomb_execute_custom", plan.handle, p_in, p_in + row, p_in + 2 * row,
+                      p_fr, p_pow, C.c_int32(0), p_ws, s)
+        else:
+            _lib.call("gls_lomb_execute", plan.handle, p_in, p_in + row, p_in + 2 * row,
+                      p_pow, p_ws, s)
+        if k:
+            p_top = self.arena.ptr(blocks[-1])
+            _lib.call("gls_topk", plan.handle, p_ws, p_pow, None, C.c_int32(k), p_top,
+                      p_top + 4 * nreq * k, s)
+            _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_top,
+                      C.c_size_t(4 * nreq * k), s)
+            _lib.call("gls_memcpy_d2h_async", slot.h_idx.ctypes.data, p_top + 4 * nreq * k,
+                      C.c_size_t(4 * nreq * k), s)
+        else:
+            _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_pow, C.c_size_t(4 * out), s)
         slot.event.record(slot.stream)
-        slot.batch = _Batch(reqs, plan, blocks, obs, out, offsets)
+        slot.batch = _Batch(reqs, plan, blocks, obs, out, offsets, k=k)
         self.batches_launched += 1
         return True
 
@@ -300,3 +380,57 @@ class GLSServer:
             "batches": self.batches_launched,
             "requests_obj": reqs,
         }
+
+    # -- record streams ----------------------------------------------------------
+    def serve_records(self, reader, writer, sleep=5e-5):
+        """Serve request records read from the binary stream ``reader`` (until
+        EOF) and write one result record per request to ``writer``, in
+        completion order.  Returns the number of results written."""
+        written = 0
+
+        def flush():
+            nonlocal written
+            for r in self.completed:
+                writer.write(wire.encode_result(_result_record(r)))
+                written += 1
+            self.completed.clear()
+
+        for rec in wire.iter_requests(reader):
+            if rec.freqs is not None:
+                self.submit(rec.t, rec.y, rec.dy, rec.freqs, custom_freqs=True, k=rec.k,
+                            mode=rec.mode, rid=rec.id)
+            else:
+                if rec.nf < 1 or not (rec.df > 0):
+                    raise wire.WireError(f"request {rec.id}: bad linear grid")
+                self.submit(rec.t, rec.y, rec.dy, rec.frequencies(), k=rec.k, mode=rec.mode,
+                            rid=rec.id)
+            self.poll()
+            flush()
+        while self.busy:
+            if self.poll() == 0:
+                time.sleep(sleep)
+            flush()
+        flush()
+        return written
+
+
+def _fap(r, z, idx):
+    """Baluev false-alarm probability of each peak (fap_baluev,
+    cuvarbase/lombscargle.py:1115-1184, with fmax the grid's highest frequency)."""
+    from .lombscargle import fap_baluev
+
+    fmax = float(np.max(r.freqs)) if r.freqs is not None else r.f0 + r.df * (r.nf - 1)
+    z = np.asarray(z, dtype=np.float64)
+    out = np.full(len(z), np.nan)
+    ok = np.asarray(idx) >= 0
+    if ok.any():
+        out[ok] = fap_baluev(r.t - np.mean(r.t), r.dy, z[ok], fmax)
+    return out
+
+
+def _result_record(r):
+    lat = r.latency if r.latency is not None else float("nan")
+    if r.top_power is not None:
+        return wire.ResultRecord(r.id, top_power=r.top_power, top_index=r.top_index,
+                                 top_freq=r.top_freq, fap=r.fap, latency=lat)
+    return wire.ResultRecord(r.id, power=r.power, latency=lat)

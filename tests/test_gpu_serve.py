@@ -60,3 +60,52 @@ def test_request_larger_than_budget_raises():
     srv.submit(t, y, dy, freqs[0])
     with pytest.raises(MemoryError):
         srv.drain()
+
+
+def test_serve_records_mixed_classes():
+    """Linear full-periodogram, linear top-k and custom-frequency requests in
+    one record stream: each result equals the same request computed alone."""
+    import io
+
+    from cuvarbase_amd import engine, wire
+    from cuvarbase_amd.lombscargle import fap_baluev
+    from cuvarbase_amd.serve import GLSServer
+
+    _, data, freqs = _stream(n=9, seed=44)
+    recs = []
+    rng = np.random.default_rng(5)
+    for i, ((t, y, dy), f) in enumerate(zip(data, freqs)):
+        if i % 3 == 0:
+            recs.append(wire.RequestRecord(100 + i, t, y, dy, f0=f[0], df=f[1] - f[0],
+                                           nf=len(f)))
+        elif i % 3 == 1:
+            recs.append(wire.RequestRecord(100 + i, t, y, dy, f0=f[0], df=f[1] - f[0],
+                                           nf=len(f), k=5))
+        else:
+            recs.append(wire.RequestRecord(100 + i, t, y, dy,
+                                           freqs=np.sort(rng.uniform(0.01, 5.0, 2000)), k=0))
+    src = io.BytesIO(b"".join(wire.encode_request(r) for r in recs))
+    dst = io.BytesIO()
+    srv = GLSServer(hbm_budget=4e9, slots=2, max_batch_evals=5e6)
+    assert srv.serve_records(src, dst) == len(recs)
+    dst.seek(0)
+    res = {r.id: r for r in wire.iter_results(dst)}
+    assert sorted(res) == [r.id for r in recs]
+    for r in recs:
+        out = res[r.id]
+        assert out.status == 0 and out.latency > 0
+        if r.freqs is not None:
+            alone = engine.LombBatch([(r.t, r.y, r.dy)], [r.freqs], custom_freqs=True).run()[0]
+            np.testing.assert_allclose(out.power, alone, atol=1e-6)
+            continue
+        alone = engine.LombBatch([(r.t, r.y, r.dy)], [r.frequencies()], sigma=srv.sigma,
+                                 m=srv.m).run()[0]
+        if r.k == 0:
+            assert np.max(np.abs(out.power - alone)) < 1e-6
+        else:
+            order = np.argsort(-alone, kind="stable")[:5]
+            np.testing.assert_array_equal(out.top_index[:1], order[:1])
+            np.testing.assert_allclose(out.top_power, alone[out.top_index], atol=1e-6)
+            np.testing.assert_allclose(out.top_freq, r.frequencies()[out.top_index])
+            fap = fap_baluev(r.t, r.dy, out.top_power.astype(np.float64), r.frequencies()[-1])
+            np.testing.assert_allclose(out.fap, fap, rtol=1e-9, atol=1e-12)
