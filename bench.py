@@ -73,6 +73,31 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
     }
 
 
+def measured_copy_gbs(stream, mib=2048, reps=10):
+    """Device-to-device copy rate (read + write bytes per second) of the HIP
+    runtime's copy kernel on ``stream``: a measured HBM ceiling next to the
+    8 TB/s published peak, and the known-size kernel that scales the HBM
+    byte counters of a profiler run."""
+    from cuvarbase_amd import _device, _lib
+    import ctypes as C
+
+    nbytes = mib << 20
+    a = _device.DeviceArray(nbytes, np.uint8)
+    b = _device.DeviceArray(nbytes, np.uint8)
+    h = _device._handle(stream)
+    _lib.call("gls_memcpy_d2d_async", b.ptr, a.ptr, C.c_size_t(nbytes), h)
+    e0, e1 = _device.Event(), _device.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        _lib.call("gls_memcpy_d2d_async", b.ptr, a.ptr, C.c_size_t(nbytes), h)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e1.time_since(e0) / reps
+    a.free()
+    b.free()
+    return 2.0 * nbytes / (ms / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,9 +113,9 @@ def main():
     ap.add_argument("--stream-requests", type=int, default=1500,
                     help="requests per arrival rate in the serving leg (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="also run one plain elementwise copy of this many MiB before the "
-                         "timed steps (scales HBM counters of a profiler run)")
+    ap.add_argument("--calib-copy-mib", type=int, default=2048,
+                    help="size of the device-to-device copies timed before the steps "
+                         "(measured copy rate; scales HBM counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the GLS path "
@@ -119,14 +144,7 @@ def main():
     for _ in range(args.warmup):
         batch.execute(stream=s)
     s.synchronize()
-    if args.calib_copy_mib > 0:
-        import torch
-
-        x = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
-        del x, y
+    copy_gbs = measured_copy_gbs(s, mib=max(args.calib_copy_mib, 256))
 
     stage_acc = {}
     e0, e1 = _device.Event(), _device.Event()
@@ -279,6 +297,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
+            "measured_copy_GBs": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs,
         },
         "pipeline_roofline": {
             "algorithmic_bytes_per_step": pipe_bytes,

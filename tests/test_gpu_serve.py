@@ -1,8 +1,17 @@
-"""Continuous-batching server on the GPU."""
+"""Continuous-batching server on the GPU.
+
+A request's periodogram served in a mix and computed alone differ only by the
+order of FP32 additions in the LDS spreading; the deconvolution multiplies
+that round-off by up to exp(b pi^2 / (4 sigma^2)) (~100 at the tuned
+sigma = 1.45, m = 8), so the two agree to MIX_TOL = 2e-5, well inside the
+1e-4 parity bar against FP64.
+"""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+MIX_TOL = 2e-5
 
 
 def _stream(n=40, rate=2000.0, seed=41):
@@ -26,7 +35,7 @@ def test_each_result_same_alone_and_in_the_mix():
         worst = max(worst, float(np.max(np.abs(r.power.astype(float) - alone))))
     print(f"served {st['requests']} requests in {st['batches']} batches, p50 {st['p50_ms']:.1f} "
           f"ms, p99 {st['p99_ms']:.1f} ms; alone vs mix max|dP| = {worst:.2e}")
-    assert worst < 1e-6
+    assert worst < MIX_TOL
 
 
 def test_admission_waits_for_memory():
@@ -35,11 +44,15 @@ def test_admission_waits_for_memory():
 
     arrivals, data, freqs = _stream(n=12, seed=42)
     # budget for about two of the largest requests at a time
+    from cuvarbase_amd.tuning import choose_nfft_params
+
+    sigma, m = choose_nfft_params(1e-4)
     sizes = []
     for d, f in zip(data, freqs):
-        b = LombBatch([d], [f], sigma=1.2, m=18)
+        b = LombBatch([d], [f], sigma=sigma, m=m)
         sizes.append(b.plan.workspace_bytes + 24 * len(d[0]) + 4 * len(f) + 3 * 256)
-    srv = GLSServer(hbm_budget=2.2 * max(sizes), slots=4, max_batch_evals=1e9, sigma=1.2, m=18)
+    srv = GLSServer(hbm_budget=2.2 * max(sizes), slots=4, max_batch_evals=1e9, sigma=sigma,
+                    m=m)
     for (t, y, dy), f in zip(data, freqs):
         srv.submit(t, y, dy, f)
     peak = 0
@@ -55,7 +68,7 @@ def test_request_larger_than_budget_raises():
     from cuvarbase_amd.serve import GLSServer
 
     _, data, freqs = _stream(n=1, seed=43)
-    srv = GLSServer(hbm_budget=1 << 20, slots=1)
+    srv = GLSServer(hbm_budget=4096, slots=1)
     t, y, dy = data[0]
     srv.submit(t, y, dy, freqs[0])
     with pytest.raises(MemoryError):
@@ -101,11 +114,12 @@ def test_serve_records_mixed_classes():
         alone = engine.LombBatch([(r.t, r.y, r.dy)], [r.frequencies()], sigma=srv.sigma,
                                  m=srv.m).run()[0]
         if r.k == 0:
-            assert np.max(np.abs(out.power - alone)) < 1e-6
+            assert np.max(np.abs(out.power - alone)) < MIX_TOL
         else:
-            order = np.argsort(-alone, kind="stable")[:5]
-            np.testing.assert_array_equal(out.top_index[:1], order[:1])
-            np.testing.assert_allclose(out.top_power, alone[out.top_index], atol=1e-6)
+            # the served best peak is the best alone (up to MIX_TOL ties)
+            assert alone[out.top_index[0]] >= alone.max() - MIX_TOL
+            assert np.all(np.diff(out.top_power) <= 0)
+            np.testing.assert_allclose(out.top_power, alone[out.top_index], atol=MIX_TOL)
             np.testing.assert_allclose(out.top_freq, r.frequencies()[out.top_index])
             fap = fap_baluev(r.t, r.dy, out.top_power.astype(np.float64), r.frequencies()[-1])
             np.testing.assert_allclose(out.fap, fap, rtol=1e-9, atol=1e-12)
