@@ -73,6 +73,22 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
     }
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` (read + write, scaled by the copy
+    kernel) from the committed rocprofv3 PMC summary of this workload, or
+    None if the summary does not list it."""
+    import re
+
+    path = os.path.join(HERE, "profiles", "survey5000_passes_kernel_stats.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch: (.+?) = (\d+)\s*$", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(2))
+    return None
+
+
 def measured_copy_gbs(stream, mib=2048, reps=10):
     """Device-to-device copy rate (read + write bytes per second) of the HIP
     runtime's copy kernel on ``stream``: a measured HBM ceiling next to the
@@ -113,6 +129,8 @@ def main():
     ap.add_argument("--stream-requests", type=int, default=1500,
                     help="requests per arrival rate in the serving leg (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--grid-chunk-mib", type=int, default=0,
+                    help="NFFT grid chunk size in MiB (0: no chunking)")
     ap.add_argument("--calib-copy-mib", type=int, default=2048,
                     help="size of the device-to-device copies timed before the steps "
                          "(measured copy rate; scales HBM counters of a profiler run)")
@@ -136,7 +154,8 @@ def main():
     from cuvarbase_amd import engine, _device
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
 
-    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m)
+    chunk = args.grid_chunk_mib << 20 if args.grid_chunk_mib > 0 else 0
+    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk)
     batch.plan.enable_timing(True)
     s = _device.Stream()
     batch.upload(stream=s)
@@ -175,7 +194,9 @@ def main():
         acc += ms
         if name == "pass2":
             ends[p] = acc
-    geo = [batch.plan.geometry(i) for i in range(batch.ncurves)]
+    ncurves = batch.ncurves
+    ws_bytes = batch.plan.workspace_bytes
+    geo = [batch.plan.geometry(i) for i in range(ncurves)]
     lat = [ends.get(g[0], acc) for g in geo]
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
@@ -198,6 +219,7 @@ def main():
     dom = max(by_kernel, key=lambda k: by_kernel[k][1])
     c, ms, by = by_kernel[dom]
     achieved = (by / c) / (ms / c / 1e3) / 1e9
+    traffic = profiled_traffic(dom)
 
     # FP64 parity on a sample (independent high-accuracy oracle NFFT)
     from oracle import nfft as onfft
@@ -279,13 +301,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"survey-{batch.ncurves}: N log-uniform [50,5000], baselines "
+            "workload": f"survey-{ncurves}: N log-uniform [50,5000], baselines "
                         "U[30,3650] d, seasonal gaps, df=1/(5T), fmax=24/d",
-            "curves": batch.ncurves,
+            "curves": ncurves,
             "evals_per_step": total_evals,
             "sigma": sigma,
             "m": m,
             "grid_cells_per_step": int(3 * sum(g[0] for g in geo)),
+            "grid_chunk_bytes": int(chunk),
+            "workspace_bytes": int(ws_bytes),
         },
         "roofline": {
             "bound": "hbm",
@@ -296,7 +320,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/survey5000_passes_kernel_stats.md (rocprofv3 --pmc "
+                              "FETCH_SIZE, WRITE_SIZE; scaled by the copy kernel)"
+                              if traffic is not None else None,
+            "algorithmic_bytes_per_launch": by / c,
             "measured_copy_GBs": copy_gbs,
             "frac_of_measured_copy": achieved / copy_gbs,
         },

@@ -11,7 +11,8 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgls.so")
+# the in-tree build; CUVARBASE_AMD_LIBGLS names another build of the same ABI
+LIB_PATH = os.environ.get("CUVARBASE_AMD_LIBGLS") or os.path.join(HERE, "libgls.so")
 
 GLS_OK = 0
 GLS_E_INVALID = -1
@@ -58,7 +59,8 @@ class gls_curve(C.Structure):
 class gls_params(C.Structure):
     _fields_ = [("sigma", C.c_double), ("m", C.c_int32), ("mode", C.c_int32),
                 ("reg_c", C.c_double), ("reg_s", C.c_double),
-                ("input_kind", C.c_int32), ("method", C.c_int32)]
+                ("input_kind", C.c_int32), ("method", C.c_int32),
+                ("grid_chunk_bytes", C.c_int64)]
 
 
 class gls_nfft_item(C.Structure):

@@ -6,6 +6,20 @@
 namespace gls {
 
 constexpr int kWave = 64;
+constexpr int kNumXcd = 8;   // MI355X: 8 accelerator complex dies, one L2 each
+
+// The dispatcher hands consecutive workgroups to the XCDs in turn.  This maps
+// the hardware index to a logical one so that each XCD runs a contiguous range
+// of logical blocks: neighbouring tiles (which share cache lines of the
+// intermediate grid and the same point lists) then meet in the same L2.
+// A bijection of [0, nblocks) for any nblocks.
+__host__ __device__ constexpr int xcd_logical_block(int bid, int nblocks) {
+    const int q = nblocks / kNumXcd;
+    const int r = nblocks % kNumXcd;
+    const int x = bid % kNumXcd;
+    const int i = bid / kNumXcd;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

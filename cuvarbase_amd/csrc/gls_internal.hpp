@@ -89,9 +89,11 @@ constexpr int pass1_cols(int n1) {
     while (c < 16 && n1 * c * 2 <= 4096) c *= 2;
     return c;
 }
-// pass-2 rows per tile: power of two with n2 * cr = 2048 where possible
+// pass-2 rows per tile (power of two): tiles of n2 * cr = 1024 row outputs for
+// rows of up to 256 (more, smaller blocks in flight), 2048 for longer rows
+// (fewer, wider row segments per read); measured on the survey batch
 constexpr int pass2_rows(int p2) {
-    int c = 2048 >> p2;
+    int c = (p2 <= 8 ? 1024 : 2048) >> p2;
     if (c > 16) c = 16;
     if (c < 1) c = 1;
     return c;

@@ -150,8 +150,20 @@ __device__ __forceinline__ float2 tw_lookup(const TwTable<L>& T, int x) {
     return l == 0 ? h : cmul(h, T.lo[l]);
 }
 
+// LDS placement of element i of interleaved sequence b: flattened index
+// f = i*BS + b, plus one pad slot after every 16 elements.  The pad breaks the
+// power-of-two strides of the Stockham writes ((j - k) R + k + q NS), which
+// would otherwise put most lanes of a wave on the same LDS banks.
+template <int BS>
+__device__ __forceinline__ int lds_at(int i, int b) {
+    const int f = i * BS + b;
+    return f + (f >> 4);
+}
+// LDS elements needed for L rows of stride BS (largest lds_at index + 1)
+constexpr int lds_size(int L, int BS) { return L * BS + ((L * BS) >> 4) + 1; }
+
 // One radix-R Stockham stage over B interleaved sequences of length L held in
-// LDS as s[i*BS + b].  NS = product of the radices of earlier stages.
+// LDS at s[lds_at<BS>(i, b)].  NS = product of the radices of earlier stages.
 template <int L, int B, int BS, int R, int NS, int NT>
 __device__ __forceinline__ void stockham_stage(float2* s, const TwTable<L>& T, int tid) {
     constexpr int LR = L / R;
@@ -165,7 +177,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, const TwTable<L>& T, i
             const int b = beta % B;
             const int j = beta / B;
 #pragma unroll
-            for (int q = 0; q < R; ++q) x[r][q] = s[(j + q * LR) * BS + b];
+            for (int q = 0; q < R; ++q) x[r][q] = s[lds_at<BS>(j + q * LR, b)];
             if constexpr (NS > 1) {
                 const int k = j % NS;
                 // w = exp(2 pi i k / (NS R)); twiddle of input q is w^q
@@ -192,7 +204,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, const TwTable<L>& T, i
             const int k = j % NS;
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) s[(base + q * NS) * BS + b] = x[r][q];
+            for (int q = 0; q < R; ++q) s[lds_at<BS>(base + q * NS, b)] = x[r][q];
         }
     }
     __syncthreads();
@@ -210,7 +222,7 @@ struct LdsFftStages {
 };
 
 // Full in-place FFT (natural order in and out) of B sequences of length L
-// stored at s[i*BS + b].  The caller has synchronised after writing s and T;
+// stored at s[lds_at<BS>(i, b)].  The caller has synchronised after writing s and T;
 // returns synchronised.
 template <int L, int B, int BS, int NT>
 __device__ __forceinline__ void lds_fft(float2* s, const TwTable<L>& T, int tid) {
@@ -218,8 +230,8 @@ __device__ __forceinline__ void lds_fft(float2* s, const TwTable<L>& T, int tid)
     LdsFftStages<L, B, BS, 1, NT>::run(s, T, tid);
 }
 
-// padded LDS row stride for B interleaved sequences (odd stride avoids bank
-// conflicts when lanes walk the sequence index)
-constexpr int lds_stride(int B) { return B >= 4 ? B + 1 : B; }
+// LDS row stride for B interleaved sequences (bank spreading comes from the
+// pad slots of lds_at)
+constexpr int lds_stride(int B) { return B; }
 
 }  // namespace gls

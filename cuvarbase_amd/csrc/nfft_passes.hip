@@ -35,31 +35,20 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     constexpr int C1 = pass1_cols(N1);
     constexpr int BS = lds_stride(C1);
     constexpr int NH = (N1 + 63) / 64;
-    __shared__ float2 tile[N1 * BS];
+    __shared__ float2 tile[lds_size(N1, BS)];
     __shared__ TwTable<N1> T;
     __shared__ float2 ehi[C1 * NH];   // exp(2 pi i u2 64 kh / n)
     __shared__ float2 elo[C1 * 64];   // exp(2 pi i u2 kl / n)
     const int tid = threadIdx.x;
     const int n2 = 1 << p2;
     const int nb = n2 / C1;
-    const int bid = blockIdx.x;
+    const int bid = xcd_logical_block(blockIdx.x, gridDim.x);
     const int jb = bid % nb;
     const int gi = (bid / nb) % ngrids;
     const int ci = curves[bid / (nb * ngrids)];
     const CurveGeom& G = geom[ci];
     const int n = G.n;
     const int a = jb * C1;
-
-    for (int i = tid; i < N1 * BS; i += NT1) tile[i] = make_float2(0.f, 0.f);
-    build_twiddles<N1, NT1>(T, tid);
-    for (int i = tid; i < C1 * (NH + 64); i += NT1) {
-        const int jj = i / (NH + 64);
-        const int r = i - jj * (NH + 64);
-        const int64_t u2 = a + jj;
-        if (r < 64) elo[jj * 64 + r] = expi_ratio(u2 * r, n);
-        else ehi[jj * NH + (r - 64)] = expi_ratio(u2 * 64 * (r - 64), n);
-    }
-    __syncthreads();
 
     // point set: buckets jb-nbk .. jb (cyclic), at most two contiguous ranges
     const int g = (gi == 2) ? 2 : 1;
@@ -82,15 +71,43 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     const int32_t* Ua = (g == 1) ? pa.U1 : pa.U2;
     const float* Fa = (g == 1) ? pa.fr1 : pa.fr2;
     const float2* Va = (gi == 0) ? pa.v0 : (gi == 1 ? pa.v1 : pa.v2);
+
+    // the first PF points of each thread are loaded before the tile is
+    // cleared and the twiddle tables are built, so their latency overlaps
+    // that work
+    constexpr int PF = 2;
+    int pU[PF];
+    float pfr[PF];
+    float2 pv[PF];
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+        const int q = tid + f * NT1;
+        pU[f] = 0;
+        pfr[f] = 0.f;
+        pv[f] = make_float2(0.f, 0.f);
+        if (q < total) {
+            const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
+            pU[f] = Ua[o];
+            pfr[f] = Fa[o];
+            pv[f] = Va[o];
+        }
+    }
+
+    for (int i = tid; i < lds_size(N1, BS); i += NT1) tile[i] = make_float2(0.f, 0.f);
+    build_twiddles<N1, NT1>(T, tid);
+    for (int i = tid; i < C1 * (NH + 64); i += NT1) {
+        const int jj = i / (NH + 64);
+        const int r = i - jj * (NH + 64);
+        const int64_t u2 = a + jj;
+        if (r < 64) elo[jj * 64 + r] = expi_ratio(u2 * r, n);
+        else ehi[jj * NH + (r - 64)] = expi_ratio(u2 * 64 * (r - 64), n);
+    }
+    __syncthreads();
+
     const float binv = (float)(1.4426950408889634 / G.b);   // log2(e)/b
     const float norm = (float)(1.0 / sqrt(M_PI * G.b));
     const int two_m = 2 * m;
-    for (int q = tid; q < total; q += NT1) {
-        const int idx = q < len0 ? s0 + q : s1 + (q - len0);
-        const int64_t o = G.pts + idx;
-        const int U = Ua[o];
-        const float fr = Fa[o];
-        const float2 v = Va[o];
+    auto spread = [&](int U, float fr, float2 v) {
         const int col0 = U & (n2 - 1);
 #pragma unroll
         for (int jj = 0; jj < C1; ++jj) {
@@ -101,10 +118,18 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
                 const int u1 = cell >> p2;
                 const float d = (float)(m - k) + fr;
                 const float wgt = exp2f(-d * d * binv) * norm;
-                atomicAdd(&tile[u1 * BS + jj].x, v.x * wgt);
-                atomicAdd(&tile[u1 * BS + jj].y, v.y * wgt);
+                float2* cellp = &tile[lds_at<BS>(u1, jj)];
+                atomicAdd(&cellp->x, v.x * wgt);
+                atomicAdd(&cellp->y, v.y * wgt);
             }
         }
+    };
+#pragma unroll
+    for (int f = 0; f < PF; ++f)
+        if (tid + f * NT1 < total) spread(pU[f], pfr[f], pv[f]);
+    for (int q = tid + PF * NT1; q < total; q += NT1) {
+        const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
+        spread(Ua[o], Fa[o], Va[o]);
     }
     __syncthreads();
 
@@ -116,7 +141,7 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
         const int jj = i / N1;
         const float2 eh = ehi[jj * NH + (k1 >> 6)];
         const float2 w = (k1 & 63) ? cmul(eh, elo[jj * 64 + (k1 & 63)]) : eh;
-        out[(int64_t)(a + jj) * N1 + k1] = cmul(tile[k1 * BS + jj], w);
+        out[(int64_t)(a + jj) * N1 + k1] = cmul(tile[lds_at<BS>(k1, jj)], w);
     }
 }
 
@@ -124,39 +149,59 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
 // pass 2: row FFT + deconvolution + GLS assembly (or NFFT output)
 // ---------------------------------------------------------------------------
 // OUT: 0 = f32 power, 1 = f64 power, 2 = complex64 NFFT (single grid)
-template <int OUT>
-constexpr int pass2_threads() { return OUT == 2 ? 128 : 192; }
+// threads per pass-2 block: one radix-16 butterfly per thread in the first
+// stages (a 2048-output tile x 3 grids holds 384 butterflies, a 1024-output
+// tile 192)
+template <int P2, int OUT>
+constexpr int pass2_threads() { return OUT == 2 ? 128 : (P2 >= 9 ? 384 : 192); }
 
 template <int P2, int OUT>
-__global__ __launch_bounds__(pass2_threads<OUT>()) void pass2_kernel(
+__global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const CurveGeom* __restrict__ geom, const CurveState* __restrict__ state,
     const int32_t* __restrict__ curves, int n1, const float2* __restrict__ grids,
     PlanParams pp, void* __restrict__ out) {
-    constexpr int NT = pass2_threads<OUT>();
+    constexpr int NT = pass2_threads<P2, OUT>();
     constexpr int N2 = 1 << P2;
     constexpr int CR = pass2_rows(P2);
     constexpr int NG = (OUT == 2) ? 1 : 3;
     constexpr int B = NG * CR;
     constexpr int BS = lds_stride(B);
-    __shared__ float2 tile[N2 * BS];
+    __shared__ float2 tile[lds_size(N2, BS)];
     __shared__ TwTable<N2> T;
     const int tid = threadIdx.x;
     const int nrb = (n1 + CR - 1) / CR;   // the last row block may be partial
-    const int bid = blockIdx.x;
+    const int bid = xcd_logical_block(blockIdx.x, gridDim.x);
     const int rb = bid % nrb;
     const int ci = curves[bid / nrb];
     const CurveGeom& G = geom[ci];
     const int n = G.n;
     const int r0 = rb * CR;
 
+    // load the tile: all of a thread's loads are issued before any is
+    // written to LDS (up to LOAD_BATCH in flight per thread)
     const float2* src = grids + G.grid;
-    for (int i = tid; i < N2 * B; i += NT) {
-        const int bb = i % B;
-        const int u2 = i / B;
-        const int gi = bb / CR;
-        const int r = bb - gi * CR;
-        tile[u2 * BS + bb] = (r0 + r < n1) ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
-                                           : make_float2(0.f, 0.f);
+    constexpr int TOTAL = N2 * B;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
+    constexpr int LOAD_BATCH = PER < 16 ? PER : 16;
+#pragma unroll
+    for (int q0 = 0; q0 < PER; q0 += LOAD_BATCH) {
+        float2 buf[LOAD_BATCH];
+#pragma unroll
+        for (int q = 0; q < LOAD_BATCH; ++q) {
+            const int i = tid + (q0 + q) * NT;
+            const int bb = i % B;
+            const int u2 = i / B;
+            const int gi = bb / CR;
+            const int r = bb - gi * CR;
+            buf[q] = (i < TOTAL && q0 + q < PER && r0 + r < n1)
+                         ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
+                         : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < LOAD_BATCH; ++q) {
+            const int i = tid + (q0 + q) * NT;
+            if (i < TOTAL && q0 + q < PER) tile[lds_at<BS>(i / B, i % B)] = buf[q];
+        }
     }
     build_twiddles<N2, NT>(T, tid);
     __syncthreads();
@@ -181,15 +226,15 @@ __global__ __launch_bounds__(pass2_threads<OUT>()) void pass2_kernel(
         const float dc = expf(bpi2 * xi * xi);
         const int k = lp + kc;
         if (OUT == 2) {
-            const float2 V = cscale(tile[k2 * BS + r], dc);
+            const float2 V = cscale(tile[lds_at<BS>(k2, r)], dc);
             // restore the absolute time origin: exp(2 pi i f_k tref)
             const double fk = G.f0 + (double)k * G.df;
             const float2 ph = expi_rev(fk * st.tref);
             ((float2*)out)[G.out + k] = cmul(V, ph);
         } else {
-            const float2 syw = tile[k2 * BS + r];
-            const float2 w1 = tile[k2 * BS + CR + r];
-            const float2 w2 = tile[k2 * BS + 2 * CR + r];
+            const float2 syw = tile[lds_at<BS>(k2, r)];
+            const float2 w1 = tile[lds_at<BS>(k2, CR + r)];
+            const float2 w2 = tile[lds_at<BS>(k2, 2 * CR + r)];
             const double d = (double)dc;
             const double P = gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x,
                                        d * syw.y, st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
@@ -220,7 +265,7 @@ hipError_t pass2_dispatch(int n1, const CurveGeom* d_geom, const CurveState* sta
     const int64_t nblocks = (int64_t)ncurves * ((n1 + CR - 1) / CR);
     if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((pass2_kernel<P2, OUT>), dim3((unsigned)nblocks),
-                       dim3(pass2_threads<OUT>()), 0, s, d_geom, state, curves, n1, grids, pp,
+                       dim3(pass2_threads<P2, OUT>()), 0, s, d_geom, state, curves, n1, grids, pp,
                        out);
     return hipGetLastError();
 }

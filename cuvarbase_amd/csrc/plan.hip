@@ -32,7 +32,9 @@ struct gls_plan {
         int64_t first = 0;  // offset (curves) into the bucket curve list
         int count = 0;
     };
-    std::vector<Bucket> buckets;
+    std::vector<Bucket> buckets;   // one entry per chunk of a size bucket
+    std::vector<int32_t> bcurves;  // curve indices of the entries, in order
+    int64_t chunk_bytes = 0;       // > 0: grid region reused by every entry
     int ntiles = 0;         // direct method
     int rf = 1;
     // host image of the descriptors uploaded into workspace[0 .. meta.size())
@@ -117,12 +119,16 @@ bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
     return found;
 }
 
-// planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft)
+// planning of the NFFT grids of a batch; ngrids = 3 (lomb) or 1 (nfft).
+// Curves with the same (n1, p2) form a size bucket; with chunking, a bucket is
+// cut into entries of at most chunk_bytes of grids, every entry's grids start
+// at offset 0 of one shared region (entries run one after the other on the
+// stream), and the region is only as large as the largest entry.
 int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
                        std::vector<int32_t>& bcurves) {
     const int m = P->pp.m;
     std::map<std::pair<int, int>, std::vector<int32_t>> by_size;
-    int64_t bkt_ints = 0, grid_elems = 0;
+    int64_t bkt_ints = 0;
     for (int i = 0; i < P->ncurves; ++i) {
         CurveGeom& G = P->geom[i];
         const int64_t need = (int64_t)std::ceil(P->sigma * (double)G.nf);
@@ -142,10 +148,37 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         G.b = gauss_b((double)n / (double)G.nf, m);
         G.bkt = bkt_ints;
         bkt_ints += 2 * (n2 / c1 + 1);
-        G.grid = grid_elems;
-        grid_elems += (int64_t)ngrids * n;
         by_size[{n1, p2}].push_back(i);
     }
+    const int64_t chunk = P->chunk_bytes;
+    int64_t grid_total = 0, grid_region = 0;
+    for (auto& kv : by_size) {
+        gls_plan::Bucket B;
+        B.n1 = kv.first.first;
+        B.p2 = kv.first.second;
+        B.c1 = pass1_cols(B.n1);
+        B.first = (int64_t)bcurves.size();
+        B.count = 0;
+        int64_t acc = 0;   // grid elements of the open entry
+        for (int32_t ci : kv.second) {
+            CurveGeom& G = P->geom[ci];
+            const int64_t cells = (int64_t)ngrids * G.n;
+            if (chunk > 0 && B.count > 0 && (acc + cells) * (int64_t)sizeof(float2) > chunk) {
+                P->buckets.push_back(B);
+                B.first = (int64_t)bcurves.size();
+                B.count = 0;
+                acc = 0;
+            }
+            G.grid = (chunk > 0) ? acc : grid_total;
+            acc += cells;
+            grid_total += cells;
+            grid_region = std::max(grid_region, acc);
+            bcurves.push_back(ci);
+            ++B.count;
+        }
+        P->buckets.push_back(B);
+    }
+    const int64_t grid_elems = (chunk > 0) ? grid_region : grid_total;
     P->off_U1 = L.take(sizeof(int32_t) * total_pts);
     P->off_fr1 = L.take(sizeof(float) * total_pts);
     P->off_v0 = L.take(sizeof(float2) * total_pts);
@@ -157,16 +190,7 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
     }
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
-    for (auto& kv : by_size) {
-        gls_plan::Bucket B;
-        B.n1 = kv.first.first;
-        B.p2 = kv.first.second;
-        B.c1 = pass1_cols(B.n1);
-        B.first = (int64_t)bcurves.size();
-        B.count = (int)kv.second.size();
-        bcurves.insert(bcurves.end(), kv.second.begin(), kv.second.end());
-        P->buckets.push_back(B);
-    }
+    P->bcurves = bcurves;
     return GLS_OK;
 }
 
@@ -209,8 +233,8 @@ void build_stage_list(gls_plan* P) {
     if (P->pp.method == GLS_METHOD_NFFT) {
         for (const auto& B : P->buckets) {
             double b1 = 0.0, b2 = 0.0;
-            for (auto& G : P->geom) {
-                if (G.n1 != B.n1 || G.p2 != B.p2) continue;
+            for (int q = 0; q < B.count; ++q) {
+                const CurveGeom& G = P->geom[P->bcurves[B.first + q]];
                 const double n = (double)G.n;
                 b1 += ngr * (16.0 * G.n_obs + 8.0 * n);
                 b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : 4.0) * G.nf;
@@ -380,6 +404,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.method = prm.method;
     P->pp.kind = 0;
     P->sigma = prm.sigma;
+    P->chunk_bytes = std::max<int64_t>(prm.grid_chunk_bytes, 0);
     P->ncurves = ncurves;
     P->geom.resize(ncurves);
     int64_t pts = 0;
@@ -462,6 +487,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     P->pp.method = GLS_METHOD_NFFT;
     P->pp.kind = 1;
     P->sigma = sigma;
+    P->chunk_bytes = 0;
     P->ncurves = nitems;
     P->geom.resize(nitems);
     int64_t pts = 0;

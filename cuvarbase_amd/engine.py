@@ -32,7 +32,8 @@ class Plan:
         self.handle = handle
 
     @classmethod
-    def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method):
+    def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method,
+             grid_chunk_bytes=0):
         _lib.init(0)
         n = len(curves)
         arr = (_lib.gls_curve * max(n, 1))()
@@ -40,7 +41,8 @@ class Plan:
             arr[i] = c
         prm = _lib.gls_params(sigma=float(sigma), m=int(m), mode=int(mode),
                               reg_c=float(reg_c), reg_s=float(reg_s),
-                              input_kind=int(input_kind), method=int(method))
+                              input_kind=int(input_kind), method=int(method),
+                              grid_chunk_bytes=int(grid_chunk_bytes))
         h = C.c_void_p()
         _lib.call("gls_lomb_plan_create", C.byref(h), arr, C.c_int32(n), C.byref(prm))
         return cls(h.value)
@@ -124,11 +126,14 @@ class LombBatch:
     freqs : list of linear frequency grids (one per curve); with
         ``custom_freqs=True`` any 1-D arrays of frequencies, evaluated by
         direct sums at exactly those values
+    grid_chunk_bytes : if > 0, NFFT grids run in chunks of this many bytes
+        that reuse one workspace region (smaller workspace, slightly slower)
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
-                 input_kind="raw", ybar=None, yy=None, custom_freqs=False):
+                 input_kind="raw", ybar=None, yy=None, custom_freqs=False,
+                 grid_chunk_bytes=0):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)
@@ -167,7 +172,8 @@ class LombBatch:
         else:
             method = _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT
             self._freqs = None
-        self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method)
+        self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method,
+                              grid_chunk_bytes)
         # host staging of the inputs
         t = np.empty(obs, dtype=np.float64)
         if self.input_kind == _lib.INPUT_RAW:

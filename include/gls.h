@@ -84,6 +84,11 @@ typedef struct gls_params {
                          /* (cuvarbase/lombscargle.py:103-118)                              */
     int32_t input_kind;  /* GLS_INPUT_*                                                     */
     int32_t method;      /* GLS_METHOD_*                                                    */
+    int64_t grid_chunk_bytes; /* > 0: the NFFT grids of a size bucket run in chunks of at
+                           most this many bytes that reuse one workspace region (workspace
+                           shrinks from all grids of the batch to the largest chunk, at a
+                           few per cent of throughput); <= 0: one region per curve.
+                           No reference counterpart. */
 } gls_params;
 
 /* One adjoint-NFFT item: sum_j y_j exp(2 pi i (f0 + k df) t_j), k < nf. */
