@@ -258,14 +258,15 @@ def main():
         from cuvarbase_amd.serve import GLSServer
 
         arr, sdata, sfreqs = synthetic.stream_requests(1.0, nreq=args.stream_requests)
-        srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m)
+        srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m,
+                        zero_copy=True)
         serving = {"requests_per_rate": args.stream_requests, "hbm_budget_gb": args.hbm_budget_gb,
                    "rates": {}}
         for rate in (100.0, 1000.0, 10000.0, float("inf")):
             nreq = args.stream_requests if rate > 500 else min(args.stream_requests,
                                                              int(rate * 10))
             a = np.zeros(nreq) if math.isinf(rate) else arr[:nreq] / rate
-            st = srv.serve_stream(a, sdata[:nreq], sfreqs[:nreq])
+            st = srv.serve_stream(a, sdata[:nreq], sfreqs[:nreq], release=True)
             srv.completed.clear()
             serving["rates"]["burst" if math.isinf(rate) else f"{rate:g}/s"] = {
                 k: st[k] for k in ("requests", "evals", "wall_s", "evals_per_s", "p50_ms",

@@ -86,6 +86,35 @@ class DeviceArena(FreeList):
         return self.buf.ptr + block[0]
 
 
+class HostArena(FreeList):
+    """FreeList over one page-locked host buffer: D2H copies land in it
+    directly and results are handed out as views (no per-batch host
+    allocation, no copy into fresh pages)."""
+
+    def __init__(self, nbytes):
+        super().__init__(nbytes)
+        self.buf = pinned_empty(self.nbytes, np.uint8)
+
+    def view(self, block, dtype, count, offset=0):
+        start = block[0] + offset
+        return self.buf[start:start + count * np.dtype(dtype).itemsize].view(dtype)
+
+    def ptr(self, block):
+        return self.buf.ctypes.data + block[0]
+
+
+class _HostHold:
+    """A result block shared by the requests of one batch."""
+
+    def __init__(self, arena, block, count):
+        self.arena, self.block, self.count = arena, block, count
+
+    def drop(self):
+        self.count -= 1
+        if self.count == 0:
+            self.arena.release(self.block)
+
+
 class Request:
     """One light curve submitted to the server.
 
@@ -118,6 +147,16 @@ class Request:
         self.done_at = None
         self.power = None
         self.top_power = self.top_index = self.top_freq = self.fap = None
+        self._hold = None
+
+    def release(self):
+        """Give a zero-copy result's host memory back to the server; the
+        result arrays must not be used afterwards."""
+        h, self._hold = self._hold, None
+        if h is not None:
+            self.power = None
+            self.top_power = self.top_index = None
+            h.drop()
 
     @property
     def latency(self):
@@ -145,19 +184,24 @@ class _Slot:
         self.batch = None
 
     def staging(self, nobs, nout, nfreq=0, nidx=0):
+        # page-locked staging grows geometrically and is kept for the slot's life
+        def grow(cur, need):
+            return max(need, 1024, 0 if cur is None else 2 * len(cur))
+
         if self.h_in is None or self.h_in.shape[1] < nobs:
-            self.h_in = pinned_empty((3, max(nobs, 1024)), np.float64)
-        if self.h_out is None or self.h_out.size < nout:
-            self.h_out = pinned_empty(max(nout, 1024), np.float32)
+            self.h_in = pinned_empty((3, grow(None if self.h_in is None else self.h_in[0],
+                                              nobs)), np.float64)
+        if nout and (self.h_out is None or self.h_out.size < nout):
+            self.h_out = pinned_empty(grow(self.h_out, nout), np.float32)
         if nfreq and (self.h_freq is None or self.h_freq.size < nfreq):
-            self.h_freq = pinned_empty(max(nfreq, 1024), np.float64)
+            self.h_freq = pinned_empty(grow(self.h_freq, nfreq), np.float64)
         if nidx and (self.h_idx is None or self.h_idx.size < nidx):
-            self.h_idx = pinned_empty(max(nidx, 1024), np.int32)
+            self.h_idx = pinned_empty(grow(self.h_idx, nidx), np.int32)
         return self.h_in, self.h_out
 
 
 class _Batch:
-    def __init__(self, reqs, plan, blocks, nobs, nout, offsets, k=0):
+    def __init__(self, reqs, plan, blocks, nobs, nout, offsets, k=0, hblock=None):
         self.reqs = reqs
         self.plan = plan
         self.blocks = blocks
@@ -165,6 +209,7 @@ class _Batch:
         self.nout = nout
         self.offsets = offsets
         self.k = k
+        self.hblock = hblock      # result block in the host arena (zero-copy mode)
         self.t_launch = time.perf_counter()
 
 
@@ -177,10 +222,16 @@ class GLSServer:
     slots : number of batches that may be in flight at once (one stream each)
     max_batch_evals : cap on curve x frequency evaluations per batch
     sigma, m : NFFT parameters (default: tuned for |dP| <= ``tol``)
+    zero_copy : results are views into a page-locked result arena of
+        ``result_buffer_bytes`` (default 4 B x max_batch_evals x (slots + 1))
+        that the D2H copies write directly; each request's memory returns to
+        the arena on ``Request.release()``.  Otherwise results are copied into
+        arrays the requests own.
     """
 
     def __init__(self, hbm_budget=16e9, slots=3, max_batch_evals=3e8, max_batch_curves=8192,
-                 sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN):
+                 sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN, zero_copy=False,
+                 result_buffer_bytes=None):
         _lib.init(0)
         s0, m0 = choose_nfft_params(tol)
         self.sigma = float(sigma if sigma is not None else s0)
@@ -189,6 +240,11 @@ class GLSServer:
         self.max_batch_evals = int(max_batch_evals)
         self.max_batch_curves = int(max_batch_curves)
         self.arena = DeviceArena(hbm_budget)
+        self.zero_copy = bool(zero_copy)
+        self.host_arena = None
+        if self.zero_copy:
+            nb = result_buffer_bytes or 4 * self.max_batch_evals * (slots + 1)
+            self.host_arena = HostArena(int(nb))
         self.slots = [_Slot() for _ in range(slots)]
         self.queue = deque()
         self.completed = []
@@ -207,16 +263,31 @@ class GLSServer:
     def _retire(self, slot):
         b = slot.batch
         now = time.perf_counter()
-        out = slot.h_out
+        nreq = len(b.reqs)
+        if b.hblock is not None:
+            hold = _HostHold(self.host_arena, b.hblock, nreq)
+            if b.k:
+                out = self.host_arena.view(b.hblock, np.float32, nreq * b.k)
+                idx = self.host_arena.view(b.hblock, np.int32, nreq * b.k, 4 * nreq * b.k)
+            else:
+                out = self.host_arena.view(b.hblock, np.float32, b.nout)
+        else:
+            hold = None
+            out = slot.h_out
+            idx = slot.h_idx
         for i, (r, o) in enumerate(zip(b.reqs, b.offsets)):
             if b.k:
                 k = r.k
-                r.top_power = np.array(out[i * b.k:i * b.k + k])
-                r.top_index = np.array(slot.h_idx[i * b.k:i * b.k + k])
+                r.top_power = out[i * b.k:i * b.k + k]
+                r.top_index = idx[i * b.k:i * b.k + k]
+                if hold is None:
+                    r.top_power = np.array(r.top_power)
+                    r.top_index = np.array(r.top_index)
                 r.top_freq = r.frequency(r.top_index)
                 r.fap = _fap(r, r.top_power, r.top_index)
             else:
-                r.power = np.array(out[o:o + r.nf])
+                r.power = out[o:o + r.nf] if hold is not None else np.array(out[o:o + r.nf])
+            r._hold = hold
             r.done_at = now
             self.completed.append(r)
             self.evals_done += r.nf
@@ -273,7 +344,16 @@ class GLSServer:
                 plan.destroy()
                 return False
             blocks.append(blk)
-        h_in, h_out = slot.staging(obs, nreq * k if k else out, out if custom else 0, nreq * k)
+        hblock = None
+        if self.zero_copy:
+            hblock = self.host_arena.alloc(8 * nreq * k if k else 4 * max(out, 1))
+            if hblock is None:
+                for b in blocks:
+                    self.arena.release(b)
+                plan.destroy()
+                return False
+        h_in, h_out = slot.staging(obs, 0 if self.zero_copy else (nreq * k if k else out),
+                                   out if custom else 0, 0 if self.zero_copy else nreq * k)
         o = 0
         for r in reqs:
             n = len(r.t)
@@ -304,14 +384,19 @@ class GLSServer:
             p_top = self.arena.ptr(blocks[-1])
             _lib.call("gls_topk", plan.handle, p_ws, p_pow, None, C.c_int32(k), p_top,
                       p_top + 4 * nreq * k, s)
-            _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_top,
-                      C.c_size_t(4 * nreq * k), s)
-            _lib.call("gls_memcpy_d2h_async", slot.h_idx.ctypes.data, p_top + 4 * nreq * k,
-                      C.c_size_t(4 * nreq * k), s)
+            if hblock is not None:
+                _lib.call("gls_memcpy_d2h_async", self.host_arena.ptr(hblock), p_top,
+                          C.c_size_t(8 * nreq * k), s)
+            else:
+                _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_top,
+                          C.c_size_t(4 * nreq * k), s)
+                _lib.call("gls_memcpy_d2h_async", slot.h_idx.ctypes.data, p_top + 4 * nreq * k,
+                          C.c_size_t(4 * nreq * k), s)
         else:
-            _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_pow, C.c_size_t(4 * out), s)
+            dst = self.host_arena.ptr(hblock) if hblock is not None else h_out.ctypes.data
+            _lib.call("gls_memcpy_d2h_async", dst, p_pow, C.c_size_t(4 * out), s)
         slot.event.record(slot.stream)
-        slot.batch = _Batch(reqs, plan, blocks, obs, out, offsets, k=k)
+        slot.batch = _Batch(reqs, plan, blocks, obs, out, offsets, k=k, hblock=hblock)
         self.batches_launched += 1
         return True
 
@@ -335,9 +420,13 @@ class GLSServer:
                 if not any(s.batch is not None for s in self.slots):
                     if len(reqs) == 1:
                         self.queue.popleft()
+                        if self.zero_copy and self.host_arena.used > 0:
+                            raise MemoryError(
+                                f"request {reqs[0].id}: the result arena is held by "
+                                "unreleased results (call Request.release())")
                         raise MemoryError(
-                            f"request {reqs[0].id} needs more device memory than the "
-                            f"server budget ({self.arena.nbytes} bytes)")
+                            f"request {reqs[0].id} needs more device or result memory than "
+                            f"the server budget ({self.arena.nbytes} bytes)")
                     # retry with a smaller batch next time
                     self.max_batch_evals = max(1, self.max_batch_evals // 2)
                 break
@@ -353,9 +442,11 @@ class GLSServer:
                 time.sleep(sleep)
 
     # -- measurement -------------------------------------------------------------
-    def serve_stream(self, arrivals, data, freqs, sleep=5e-5):
+    def serve_stream(self, arrivals, data, freqs, sleep=5e-5, release=False):
         """Replay a request stream in real time: request i arrives arrivals[i]
-        seconds after the start.  Returns a stats dict."""
+        seconds after the start.  Returns a stats dict.  ``release=True``
+        hands each result back (zero-copy mode) as soon as it completes, as a
+        consumer that has taken it would."""
         t0 = time.perf_counter()
         reqs = []
         i, n = 0, len(data)
@@ -365,7 +456,11 @@ class GLSServer:
                 t, y, dy = data[i]
                 reqs.append(self.submit(t, y, dy, freqs[i], arrival=t0 + arrivals[i]))
                 i += 1
-            if self.poll() == 0 and (i >= n or arrivals[i] > time.perf_counter() - t0):
+            done = self.poll()
+            if release and done:
+                for r in self.completed[-done:]:
+                    r.release()
+            if done == 0 and (i >= n or arrivals[i] > time.perf_counter() - t0):
                 time.sleep(sleep)
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in reqs])
@@ -392,6 +487,7 @@ class GLSServer:
             nonlocal written
             for r in self.completed:
                 writer.write(wire.encode_result(_result_record(r)))
+                r.release()
                 written += 1
             self.completed.clear()
 

@@ -123,3 +123,22 @@ def test_serve_records_mixed_classes():
             np.testing.assert_allclose(out.top_freq, r.frequencies()[out.top_index])
             fap = fap_baluev(r.t, r.dy, out.top_power.astype(np.float64), r.frequencies()[-1])
             np.testing.assert_allclose(out.fap, fap, rtol=1e-9, atol=1e-12)
+
+
+def test_zero_copy_results_and_release():
+    """Zero-copy mode: results are views into the page-locked result arena,
+    equal to the copy mode's, and release() returns the memory."""
+    from cuvarbase_amd.serve import GLSServer
+
+    arrivals, data, freqs = _stream(n=10, seed=45)
+    a = GLSServer(hbm_budget=4e9, slots=2, max_batch_evals=3e6)
+    b = GLSServer(hbm_budget=4e9, slots=2, max_batch_evals=3e6, zero_copy=True)
+    sa = a.serve_stream(arrivals * 0, data, freqs)
+    sb = b.serve_stream(arrivals * 0, data, freqs)
+    assert b.host_arena.used > 0
+    for ra, rb in zip(sa["requests_obj"], sb["requests_obj"]):
+        assert rb.power.base is not None            # a view, not a copy
+        assert np.max(np.abs(ra.power - rb.power)) < MIX_TOL
+        rb.release()
+        assert rb.power is None
+    assert b.host_arena.used == 0
