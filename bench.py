@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--stream-requests", type=int, default=1500,
                     help="requests per arrival rate in the serving leg (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--short-curves", type=int, default=100_000,
+                    help="curves in the short-curve top-5 leg (0: skip)")
     ap.add_argument("--grid-chunk-mib", type=int, default=0,
                     help="NFFT grid chunk size in MiB (0: no chunking)")
     ap.add_argument("--calib-copy-mib", type=int, default=2048,
@@ -156,7 +158,6 @@ def main():
 
     chunk = args.grid_chunk_mib << 20 if args.grid_chunk_mib > 0 else 0
     batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk)
-    batch.plan.enable_timing(True)
     s = _device.Stream()
     batch.upload(stream=s)
     s.synchronize()
@@ -165,7 +166,8 @@ def main():
     s.synchronize()
     copy_gbs = measured_copy_gbs(s, mib=max(args.calib_copy_mib, 256))
 
-    stage_acc = {}
+    # timed steps: the production launch order (size buckets forked over the
+    # library's streams)
     e0, e1 = _device.Event(), _device.Event()
     step_ms = []
     _device.synchronize()
@@ -176,29 +178,31 @@ def main():
         e1.record(s)
         e1.synchronize()
         step_ms.append(e1.time_since(e0))
-        for (name, p, ms, by) in batch.plan.stage_times():
-            a = stage_acc.setdefault((name, p), [0, 0.0, 0.0])
-            a[0] += 1
-            a[1] += ms
-            a[2] += by
     _device.synchronize()
     wall = time.perf_counter() - t_wall
     ms_step = float(np.mean(step_ms))
     value = total_evals / (ms_step / 1e3)
 
-    # per-curve completion latency within a step (all curves arrive together):
-    # a curve is done when its size bucket's pass 2 ends
-    last = batch.plan.stage_times()
-    ends, acc = {}, 0.0
-    for (name, p, ms, _by) in last:
-        acc += ms
-        if name == "pass2":
-            ends[p] = acc
+    # instrumented steps (after the timed ones): HIP events between launches
+    # on one stream give each stage's and kernel's duration, launches in
+    # series so that each is measured alone
+    batch.plan.enable_timing(True)
+    stage_acc = {}
+    n_instr = max(1, min(args.steps, 3))
+    for _ in range(n_instr):
+        batch.execute(stream=s)
+        s.synchronize()
+        for (name, p, ms, by) in batch.plan.stage_times():
+            a = stage_acc.setdefault((name, p), [0, 0.0, 0.0])
+            a[0] += 1
+            a[1] += ms
+            a[2] += by
+    batch.plan.enable_timing(False)
+    serial_ms = sum(v[1] for v in stage_acc.values()) / n_instr
+
     ncurves = batch.ncurves
     ws_bytes = batch.plan.workspace_bytes
     geo = [batch.plan.geometry(i) for i in range(ncurves)]
-    lat = [ends.get(g[0], acc) for g in geo]
-    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
     # dominant kernel: the kernel template (pass-1 column length n1 or pass-2
     # row length n2) with the most time over the timed steps
@@ -282,6 +286,42 @@ def main():
         best = max(v["evals_per_s"] for v in serving["rates"].values())
         serving["served_vs_plain"] = best / serving["plain_evals_per_s"]
 
+    # short-curve leg: 100,000 curves of 20-300 points on one shared grid of
+    # 20,000 frequencies, top-5 peaks per curve on the device (inputs resident)
+    short_leg = None
+    if args.short_curves > 0:
+        import ctypes as C
+        from cuvarbase_amd import _lib
+
+        if "batch" in locals():
+            del batch
+        _device.synchronize()
+        shd, shf = synthetic.short(args.short_curves)
+        sb = engine.LombBatch(shd, shf, sigma=sigma, m=m)
+        sb.upload(stream=s)
+        k = 5
+        tp = _device.DeviceArray(sb.ncurves * k, np.float32)
+        ti = _device.DeviceArray(sb.ncurves * k, np.int32)
+
+        def short_step():
+            sb.execute(stream=s)
+            _lib.call("gls_topk", sb.plan.handle, sb.d_ws.ptr, sb.d_power.ptr, None,
+                      C.c_int32(k), tp.ptr, ti.ptr, _device._handle(s))
+
+        short_step()
+        s.synchronize()
+        e0.record(s)
+        for _ in range(3):
+            short_step()
+        e1.record(s)
+        e1.synchronize()
+        sms = e1.time_since(e0) / 3
+        short_leg = {"curves": sb.ncurves, "evals": int(sb.total_out), "ms": sms,
+                     "evals_per_s": sb.total_out / (sms / 1e3), "top_k": k,
+                     "workload": "N~U{20..300}, t~U[0,90] d, 20,000 freqs in [0.01, 40] /d"}
+        del sb, tp, ti
+        _device.synchronize()
+
     # pipeline-level roofline: fixed algorithmic bytes B = 16N + 24G + 28nf
     nobs = np.array([len(d[0]) for d in data], dtype=np.float64)
     nfa = np.array([len(f) for f in freqs], dtype=np.float64)
@@ -315,7 +355,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
-            "launches_per_step": c // args.steps,
+            "launches_per_step": c // n_instr,
             "avg_launch_ms": ms / c,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -334,15 +374,18 @@ def main():
             "achieved_GBs": pipe_bytes / (ms_step / 1e3) / 1e9,
             "frac": pipe_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
-        "stages_ms_per_step": {k: v[1] / args.steps for k, v in by_kind.items()},
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in
+        "stages_ms_serial": {k: v[1] / n_instr for k, v in by_kind.items()},
+        "kernels_ms_serial": {k: v[1] / n_instr for k, v in
                                 sorted(by_kernel.items(), key=lambda kv: -kv[1][1])[:8]},
         "plain_evals_per_s": plain,
         "batched_vs_plain": (value / plain) if plain else None,
-        "latency_ms": {"p50": p50, "p99": p99},
+        "latency_ms": {"p50": ms_step, "p99": ms_step,
+                       "note": "every curve of the batch completes with the step"},
+        "serial_launch_order_ms": serial_ms,
         "parity": {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(idx)}",
                    "oracle": "FP64 NumPy NFFT sigma=4 m=12"},
         "serving": serving,
+        "short_top5": short_leg,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
     }

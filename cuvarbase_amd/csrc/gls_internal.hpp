@@ -28,6 +28,11 @@ int hip_fail(hipError_t e, const char* what);
 bool initialised();
 int require_init();
 
+// library-owned streams (default priority) onto which an execute forks the
+// independent size buckets of a plan; created on first use
+constexpr int kAuxStreams = 3;
+hipError_t aux_streams(hipStream_t* out);
+
 constexpr int kMaxGrid = 1 << 24;   // largest grid n (cells)
 constexpr int kMinGrid = 1 << 10;   // smallest grid n
 

@@ -33,6 +33,8 @@ struct gls_plan {
         int count = 0;
     };
     std::vector<Bucket> buckets;   // one entry per chunk of a size bucket
+    std::vector<int> bucket_lane;  // 0: the caller's stream, j > 0: auxiliary stream j-1
+    std::vector<hipEvent_t> fork_join;  // fork event + one join event per aux stream
     std::vector<int32_t> bcurves;  // curve indices of the entries, in order
     int64_t chunk_bytes = 0;       // > 0: grid region reused by every entry
     int ntiles = 0;         // direct method
@@ -191,6 +193,27 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
     P->bcurves = bcurves;
+    // lanes: buckets own disjoint grid regions unless chunked, so they may run
+    // on kAuxStreams + 1 streams at once; largest first onto the least loaded
+    P->bucket_lane.assign(P->buckets.size(), 0);
+    if (chunk <= 0 && P->buckets.size() > 1) {
+        std::vector<std::pair<double, int>> order;
+        for (size_t i = 0; i < P->buckets.size(); ++i) {
+            double cells = 0.0;
+            const auto& B = P->buckets[i];
+            for (int q = 0; q < B.count; ++q) cells += (double)P->geom[bcurves[B.first + q]].n;
+            order.push_back({-cells, (int)i});
+        }
+        std::sort(order.begin(), order.end());
+        double load[kAuxStreams + 1] = {};
+        for (auto& o : order) {
+            int best = 0;
+            for (int j = 1; j <= kAuxStreams; ++j)
+                if (load[j] < load[best]) best = j;
+            load[best] += -o.first;
+            P->bucket_lane[o.second] = best;
+        }
+    }
     return GLS_OK;
 }
 
@@ -272,6 +295,7 @@ void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
 void destroy_plan(gls_plan* P) {
     if (!P) return;
     for (auto e : P->events) hipEventDestroy(e);
+    for (auto e : P->fork_join) hipEventDestroy(e);
     delete P;
 }
 
@@ -315,18 +339,43 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
     GLS_HIP_TRY(launch_prep(d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, y_is_f32, s));
     GLS_HIP_TRY(mark());
     if (P->pp.method == GLS_METHOD_NFFT) {
-        for (const auto& B : P->buckets) {
+        // fork the buckets over the caller's and the auxiliary streams (not
+        // when timing, whose events need one serial stream), join at the end
+        bool fork = false;
+        for (int lane : P->bucket_lane) fork |= lane > 0;
+        fork &= !P->timing;
+        hipStream_t lanes[kAuxStreams + 1] = {s};
+        if (fork) {
+            GLS_HIP_TRY(aux_streams(lanes + 1));
+            if (P->fork_join.empty()) {
+                P->fork_join.resize(kAuxStreams + 1);
+                for (auto& e : P->fork_join)
+                    GLS_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+            GLS_HIP_TRY(hipEventRecord(P->fork_join[0], s));
+            for (int j = 1; j <= kAuxStreams; ++j)
+                GLS_HIP_TRY(hipStreamWaitEvent(lanes[j], P->fork_join[0], 0));
+        }
+        for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
+            const auto& B = P->buckets[bi];
+            const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
             const int32_t* curves = d_bcurves + B.first;
             GLS_HIP_TRY(launch_pass1(B.n1, d_geom, curves, B.count, ngrids, B.p2, pa, bkt, grids,
-                                     P->pp.m, s));
+                                     P->pp.m, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
                 GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              (float2*)out, s));
+                                              (float2*)out, ls));
             else
                 GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              P->pp, out, out_f64, s));
+                                              P->pp, out, out_f64, ls));
             GLS_HIP_TRY(mark());
+        }
+        if (fork) {
+            for (int j = 1; j <= kAuxStreams; ++j) {
+                GLS_HIP_TRY(hipEventRecord(P->fork_join[j], lanes[j]));
+                GLS_HIP_TRY(hipStreamWaitEvent(s, P->fork_join[j], 0));
+            }
         }
     } else if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) {
         GLS_HIP_TRY(launch_dirsum_custom(d_geom, state, d_tiles, d_tiles + P->ntiles, P->ntiles,

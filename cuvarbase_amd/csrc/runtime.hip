@@ -13,7 +13,23 @@ thread_local std::string g_err;
 std::mutex g_mu;
 bool g_init = false;
 int g_device = -1;
+hipStream_t g_aux[kAuxStreams] = {};
 }  // namespace
+
+hipError_t aux_streams(hipStream_t* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (int i = 0; i < kAuxStreams; ++i) {
+        if (!g_aux[i]) {
+            hipError_t e = hipStreamCreateWithFlags(&g_aux[i], hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                g_aux[i] = nullptr;
+                return e;
+            }
+        }
+        out[i] = g_aux[i];
+    }
+    return hipSuccess;
+}
 
 void set_error(const std::string& msg) { g_err = msg; }
 
@@ -79,6 +95,10 @@ int gls_init(int device) {
 int gls_shutdown(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_init) return GLS_OK;
+    for (auto& a : g_aux) {
+        if (a) hipStreamDestroy(a);
+        a = nullptr;
+    }
     g_init = false;
     g_device = -1;
     return GLS_OK;
