@@ -294,8 +294,8 @@ void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
 
 void destroy_plan(gls_plan* P) {
     if (!P) return;
-    for (auto e : P->events) hipEventDestroy(e);
-    for (auto e : P->fork_join) hipEventDestroy(e);
+    for (auto e : P->events) (void)hipEventDestroy(e);
+    for (auto e : P->fork_join) (void)hipEventDestroy(e);
     delete P;
 }
 

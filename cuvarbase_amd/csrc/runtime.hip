@@ -96,7 +96,7 @@ int gls_shutdown(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_init) return GLS_OK;
     for (auto& a : g_aux) {
-        if (a) hipStreamDestroy(a);
+        if (a) (void)hipStreamDestroy(a);
         a = nullptr;
     }
     g_init = false;
