@@ -63,6 +63,12 @@ class gls_params(C.Structure):
                 ("grid_chunk_bytes", C.c_int64)]
 
 
+class gls_mh_curve(C.Structure):
+    _fields_ = [("sums_offset", C.c_int64), ("out_offset", C.c_int64), ("nf", C.c_int32),
+                ("pad", C.c_int32), ("yy", C.c_double), ("ybar", C.c_double),
+                ("sum_w", C.c_double), ("lam", C.c_double)]
+
+
 class gls_nfft_item(C.Structure):
     _fields_ = [("obs_offset", C.c_int64), ("n_obs", C.c_int32), ("nf", C.c_int32),
                 ("f0", C.c_double), ("df", C.c_double), ("out_offset", C.c_int64)]
@@ -106,6 +112,7 @@ _PROTOS = {
     "gls_lomb_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_custom": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp],
+    "gls_mh_power": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32, _vp],
     "gls_nfft_plan_create": [C.POINTER(_vp), C.POINTER(gls_nfft_item), C.c_int32,
                              C.c_double, C.c_int32],
     "gls_nfft_execute": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],

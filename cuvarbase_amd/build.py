@@ -24,6 +24,7 @@ SOURCES = [
     "nfft_passes.hip",
     "dirsum.hip",
     "misc.hip",
+    "mh.hip",
     "plan.hip",
 ]
 

@@ -315,3 +315,81 @@ def gaussian_grid(t, y, nf, sigma, m, samples_per_peak=1.0, stream=None):
     out = d_g.get_async(stream=s)
     s.synchronize()
     return out
+
+
+class MultiHarmonicBatch:
+    """Multi-harmonic floating-mean GLS (H = 1 .. 4 harmonics) of many curves.
+
+    Per curve the 3H harmonic sums (yw at n f, n <= H; w at n f, n <= 2H) are
+    adjoint NFFTs of one NFFT plan; ``gls_mh_power`` then assembles and solves
+    the 2H x 2H system per frequency in FP64.  Weights, weighted mean and
+    variance are formed on the host in FP64, as LombScargleMemory.setdata does
+    (cuvarbase/lombscargle.py:235-293)."""
+
+    def __init__(self, data, freqs, nharmonics=2, sigma=2.0, m=8, amplitude_prior=None,
+                 use_double=False):
+        if not 1 <= int(nharmonics) <= 4:
+            raise ValueError("nharmonics must be in [1, 4]")
+        if len(freqs) != len(data):
+            raise ValueError("need one frequency grid per light curve")
+        if len(data) > 65535:
+            raise ValueError("at most 65535 curves per multi-harmonic batch")
+        H = self.nharmonics = int(nharmonics)
+        self.use_double = bool(use_double)
+        self.out_dtype = np.float64 if use_double else np.float32
+        lam = 0.0 if amplitude_prior is None else float(amplitude_prior) ** -2
+        items, curves, ts, ys = [], [], [], []
+        obs = out = pw = 0
+        self.nf, self.out_offsets = [], []
+        for (t, y, dy), fr in zip(data, freqs):
+            f0, df, K = linear_grid(fr)
+            t = np.asarray(t, np.float64)
+            y = np.asarray(y, np.float64)
+            w = 1.0 / np.asarray(dy, np.float64) ** 2
+            w /= np.sum(w)
+            ybar = float(np.dot(w, y))
+            yw = w * (y - ybar)
+            yy = float(np.dot(w, (y - ybar) ** 2))
+            n = len(t)
+            ts += [t, t]
+            ys += [yw, w]
+            rows = [(obs, n) for _ in range(H)] + [(obs + n, n) for _ in range(2 * H)]
+            harm = list(range(1, H + 1)) + list(range(1, 2 * H + 1))
+            curves.append(_lib.gls_mh_curve(sums_offset=out, out_offset=pw, nf=K, pad=0, yy=yy,
+                                            ybar=float(np.sum(yw)), sum_w=float(np.sum(w)),
+                                            lam=lam))
+            for (o, cnt), h in zip(rows, harm):
+                items.append(_lib.gls_nfft_item(obs_offset=o, n_obs=cnt, nf=K, f0=h * f0,
+                                                df=h * df, out_offset=out))
+                out += K
+            self.nf.append(K)
+            self.out_offsets.append(pw)
+            obs += 2 * n
+            pw += K
+        self.total_sums = out
+        self.total_out = pw
+        self.max_nf = max(self.nf) if self.nf else 0
+        self.plan = Plan.nfft(items, sigma, m)
+        self._t = np.concatenate(ts) if ts else np.zeros(1)
+        self._y = np.concatenate(ys) if ys else np.zeros(1)
+        arr = (_lib.gls_mh_curve * max(len(curves), 1))(*curves)
+        self._curves = np.frombuffer(arr, dtype=np.uint8).copy()
+        self.ncurves = len(curves)
+
+    def run(self, stream=None):
+        s = stream or Stream()
+        d_t = DeviceArray.to_device(self._t, stream=s)
+        d_y = DeviceArray.to_device(self._y, stream=s)
+        d_c = DeviceArray.to_device(self._curves, stream=s)
+        d_sums = DeviceArray(max(2 * self.total_sums, 1), np.float32)
+        d_ws = DeviceArray(self.plan.workspace_bytes, np.uint8)
+        d_p = DeviceArray(max(self.total_out, 1), self.out_dtype)
+        _lib.call("gls_nfft_execute", self.plan.handle, d_t.ptr, d_y.ptr, C.c_int32(0),
+                  d_sums.ptr, d_ws.ptr, _handle(s))
+        _lib.call("gls_mh_power", d_sums.ptr, d_c.ptr, C.c_int32(self.ncurves),
+                  C.c_int32(self.nharmonics), C.c_int32(self.max_nf), d_p.ptr,
+                  C.c_int32(1 if self.use_double else 0), _handle(s))
+        flat = pinned_empty(max(self.total_out, 1), self.out_dtype)
+        d_p.get_async(flat, stream=s, count=self.total_out)
+        s.synchronize()
+        return [np.array(flat[o:o + n]) for o, n in zip(self.out_offsets, self.nf)]

@@ -373,7 +373,9 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
 
     ``**kwargs`` go to NFFTAsyncProcess (``sigma`` default 4, ``m`` 8,
     ``autoset_m``, ``tol``, ``use_double``) and GPUAsyncProcess (``nstreams``,
-    ``device``).  ``nharmonics > 1`` raises, as in the reference.
+    ``device``).  ``nharmonics`` up to 4 runs the multi-harmonic GLS on the
+    NFFT path (the reference raises for nharmonics > 1,
+    cuvarbase/lombscargle.py:727-730); above 4 raises.
 
     Example
     -------
@@ -394,8 +396,8 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         self.use_double = self.nfft_proc.use_double
         self.memory = None
         self.nharmonics = kwargs.get('nharmonics', 1)
-        if self.nharmonics > 1:
-            raise Exception("Only 1 harmonic is supported right now")
+        if not 1 <= self.nharmonics <= 4:
+            raise Exception("nharmonics must be between 1 and 4")
 
     def _compile_and_prepare_functions(self, **kwargs):
         self.nfft_proc._compile_and_prepare_functions(**kwargs)
@@ -507,6 +509,8 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
                 raise ValueError("need one frequency array per light curve")
             return self._run_batch(data, frqs, use_fft=False, custom=True, **kwargs)
         frqs, k0s = self._frequencies(data, freqs, **kwargs)
+        if self.nharmonics > 1:
+            return self._run_multiharmonic(data, frqs, use_fft=use_fft, memory=memory, **kwargs)
         if memory is None:
             memory = self.memory
         if memory is None:
@@ -536,6 +540,24 @@ class LombScargleAsyncProcess(GPUAsyncProcess):
         flat = batch.download_async(stream=self.streams[0]) if transfer_to_host else \
             pinned_zeros(max(batch.total_out, 1), batch.out_dtype)
         return [(f, p) for f, p in zip(frqs, batch.split(flat))]
+
+    def _run_multiharmonic(self, data, frqs, use_fft=True, memory=None, **kwargs):
+        """nharmonics > 1: MultiHarmonicBatch (harmonic NFFTs + FP64 solve)."""
+        if not use_fft or memory is not None or kwargs.get('python_dir_sums', False):
+            raise NotImplementedError("nharmonics > 1 runs on the batched NFFT path only "
+                                      "(use_fft=True, no preallocated memory)")
+        if not kwargs.get('floating_mean', True) or kwargs.get('window', False):
+            raise NotImplementedError("nharmonics > 1 is the floating-mean model only")
+        from .engine import MultiHarmonicBatch
+
+        m = max(self.nfft_proc.get_m(len(f)) for f in frqs) if frqs else self.nfft_proc.m
+        b = MultiHarmonicBatch(data, frqs, nharmonics=self.nharmonics,
+                               sigma=self.nfft_proc.sigma, m=m,
+                               amplitude_prior=kwargs.get('amplitude_prior', None),
+                               use_double=self.use_double)
+        if not self.streams:
+            self._create_streams(1)
+        return [(f, p) for f, p in zip(frqs, b.run(stream=self.streams[0]))]
 
     def _launch_batch(self, data, frqs, use_fft=True, custom=False, **kwargs):
         """Plan, upload and execute one LombBatch on streams[0] (no download)."""

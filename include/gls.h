@@ -101,6 +101,18 @@ typedef struct gls_nfft_item {
     int64_t out_offset;  /* offset in complex elements */
 } gls_nfft_item;
 
+/* One curve of a multi-harmonic power evaluation (gls_mh_power). */
+typedef struct gls_mh_curve {
+    int64_t sums_offset; /* first complex element of the curve's 3H sum rows   */
+    int64_t out_offset;  /* first output element                                */
+    int32_t nf;          /* frequencies K                                       */
+    int32_t pad;
+    double  yy;          /* weighted variance YY                                */
+    double  ybar;        /* sum of yw (the reference subtracts ybar*C, ybar*S)   */
+    double  sum_w;       /* sum of the weights, c[0] (1 for normalised weights)  */
+    double  lam;         /* amplitude-prior lambda = prior^-2 on every harmonic  */
+} gls_mh_curve;
+
 typedef struct gls_plan gls_plan;   /* opaque */
 
 /* ---- context ------------------------------------------------------------ */
@@ -221,6 +233,23 @@ int gls_nfft_execute(gls_plan* plan, const double* t, const void* y,
 int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs,
                       int32_t ng, int32_t m, double b, double tmin,
                       double tmax, double spp, float* grid, void* stream);
+
+/* ---- multi-harmonic GLS (next row: nharmonics > 1) ------------------------ */
+/* Multi-harmonic floating-mean GLS power at K frequencies per curve from the
+ * harmonic sums that an NFFT plan computed: for curve i, `sums` holds 3H rows
+ * of K complex values starting at curves[i].sums_offset:
+ *   rows 0 .. H-1:   sum yw exp(2 pi i n f_k t),  n = 1 .. H
+ *   rows H .. 3H-1:  sum w  exp(2 pi i n f_k t),  n = 1 .. 2H.
+ * Builds CC, CS, SS, YC, YS as mhdirect_sums (cuvarbase/lombscargle.py:337-400),
+ * adds the amplitude prior (add_regularization, :403-447) and returns
+ * P = (2 theta.b - theta^T A theta)/YY (mhgls_from_sums, :497-536); the
+ * reference's LombScargleAsyncProcess rejects nharmonics > 1
+ * (cuvarbase/lombscargle.py:727-730) and its lomb_mh kernel is a stub
+ * (cuvarbase/kernels/lomb.cu:223-254).  `curves` is a DEVICE array;
+ * 1 <= nharm <= 4; power f32 (power_is_f64 = 0) or f64; invalid -> -1. */
+int gls_mh_power(const void* sums, const gls_mh_curve* curves, int32_t ncurves,
+                 int32_t nharm, int32_t max_nf, void* power, int32_t power_is_f64,
+                 void* stream);
 
 /* ---- peak extraction (next row: best frequencies, lombscargle.py:1099-1107) */
 /* For each curve of a plan executed into `power` with `workspace`: the top-k

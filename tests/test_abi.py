@@ -39,6 +39,7 @@ def test_ctypes_struct_layout_matches_c():
         "gls_curve": [f[0] for f in _lib.gls_curve._fields_],
         "gls_params": [f[0] for f in _lib.gls_params._fields_],
         "gls_nfft_item": [f[0] for f in _lib.gls_nfft_item._fields_],
+        "gls_mh_curve": [f[0] for f in _lib.gls_mh_curve._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
              "int main(void) {"]

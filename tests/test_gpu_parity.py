@@ -356,3 +356,53 @@ def test_grid_chunking_matches_unchunked():
     pa, pb = a.run(), b.run()
     worst = max(float(np.max(np.abs(x.astype(float) - y))) for x, y in zip(pa, pb))
     assert worst < 1e-6
+
+
+@pytest.mark.parametrize("nharm,prior", [(1, None), (2, None), (3, 1.5), (4, None)])
+def test_multiharmonic_against_oracle(nharm, prior):
+    """nharmonics = 1..4 (harmonic NFFTs + FP64 solve on the device) against
+    the FP64 oracle mh_direct_sums + mhgls_power at sampled frequencies."""
+    from cuvarbase_amd import engine
+
+    rng = np.random.default_rng(50 + nharm)
+    data = []
+    for i in range(3):
+        n = (300, 80, 1500)[i]
+        t = np.sort(rng.uniform(0, (30.0, 5.0, 200.0)[i], n))
+        ph = 2 * np.pi * t / 1.37
+        y = 2 + np.sin(ph) + 0.5 * np.cos(2 * ph + 0.3) + 0.3 * np.sin(3 * ph) \
+            + rng.normal(0, 0.2, n)
+        dy = rng.uniform(0.1, 0.3, n)
+        data.append((t, y, dy))
+    freqs = [0.01 * (5 + np.arange(2500)), 0.05 * (1 + np.arange(400)),
+             0.002 * (3 + np.arange(6000))]
+    normed = [og.normalize_light_curve(*d) for d in data]
+    P = engine.MultiHarmonicBatch(normed, freqs, nharmonics=nharm, sigma=2.0, m=8,
+                                  amplitude_prior=prior).run()
+    for (tc, yc, dyc), f, p in zip(normed, freqs, P):
+        w, yw, ybar, YY = og.prepare(yc, dyc)
+        sel = np.unique(np.concatenate([rng.choice(len(f), 40, replace=False),
+                                        [int(np.argmax(p))]]))
+        ref = np.array([og.mhgls_power(og.mh_direct_sums(tc, yw, w, f[k], nharms=nharm), YY,
+                                       amplitude_prior=prior) for k in sel])
+        assert np.max(np.abs(p[sel] - ref)) < 1e-4
+    if nharm == 1:
+        base = engine.LombBatch(normed, freqs, sigma=2.0, m=8, amplitude_prior=prior).run()
+        for p, q in zip(P, base):
+            assert np.max(np.abs(p - q)) < 1e-5
+
+
+def test_multiharmonic_drop_in():
+    from cuvarbase_amd import engine
+    from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
+
+    t, y, dy = _curve(91, n=400)
+    freqs = 0.01 * (3 + np.arange(3000))
+    proc = LombScargleAsyncProcess(sigma=2, m=8, nharmonics=2)
+    (f, p), = proc.run([(t, y, dy)], freqs=freqs)
+    proc.finish()
+    q = engine.MultiHarmonicBatch([og.normalize_light_curve(t, y, dy)], [freqs],
+                                  nharmonics=2, sigma=2, m=8).run()[0]
+    np.testing.assert_allclose(p, q, atol=1e-6)
+    with pytest.raises(NotImplementedError):
+        proc.run([(t, y, dy)], freqs=freqs, use_fft=False)

@@ -78,7 +78,8 @@ def test_process_construction_without_device():
     assert proc.nfft_proc.sigma == 2 and proc.nfft_proc.m == 8
     assert proc.real_type is np.float32
     with pytest.raises(Exception):
-        ls.LombScargleAsyncProcess(nharmonics=2)
+        ls.LombScargleAsyncProcess(nharmonics=5)
+    assert ls.LombScargleAsyncProcess(nharmonics=3).nharmonics == 3
     # autoset_m (cuvarbase/cunfft.py:378-412)
     p = ls.LombScargleAsyncProcess(autoset_m=True, sigma=4, tol=1e-8)
     assert p.nfft_proc.get_m(10000) == int(np.ceil(-np.log(0.25 * 1e-12) /

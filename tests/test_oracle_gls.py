@@ -192,3 +192,22 @@ def test_golden_single_curve_periodogram():
     P = og.gls_direct(t, y, dy, freqs[0][sel])
     np.testing.assert_allclose(P, g["power"][sel], rtol=0, atol=1e-12)
     assert math.isclose(1 / g["freqs"][np.argmax(g["power"])], 0.73, rel_tol=2e-3)
+
+
+@pytest.mark.parametrize("nharms,prior", [(3, None), (2, 0.7), (4, 2.0)])
+def test_multiharmonic_power_equals_least_squares(nharms, prior):
+    """mhgls_power (cuvarbase/lombscargle.py:337-536 restated) equals the
+    weighted least-squares fit of H harmonics + offset, with ridge rows for
+    the amplitude prior."""
+    rng = np.random.default_rng(31 + nharms)
+    t = np.sort(rng.uniform(0, 20, 120))
+    y = 1.0 + np.sin(2 * np.pi * t / 2.3) + 0.4 * np.cos(4 * np.pi * t / 2.3) \
+        + rng.normal(0, 0.3, 120)
+    dy = rng.uniform(0.2, 0.5, 120)
+    tc, yc, dyc = og.normalize_light_curve(t, y, dy)
+    w, yw, ybar, YY = og.prepare(yc, dyc)
+    for f in (0.17, 1 / 2.3, 1.9):
+        s = og.mh_direct_sums(tc, yw, w, f, nharms=nharms)
+        got = og.mhgls_power(s, YY, amplitude_prior=prior)
+        ref = og.lstsq_power(tc, yc, dyc, f, nharms=nharms, amplitude_prior=prior)
+        assert got == pytest.approx(ref, rel=1e-8, abs=1e-12)
