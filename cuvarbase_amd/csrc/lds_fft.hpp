@@ -164,59 +164,117 @@ constexpr int lds_size(int L, int BS) { return L * BS + ((L * BS) >> 4) + 1; }
 
 // One radix-R Stockham stage over B interleaved sequences of length L held in
 // LDS at s[lds_at<BS>(i, b)].  NS = product of the radices of earlier stages.
+// Butterfly beta (b = beta % B, j = beta / B) reads x[j + q L/R], multiplies
+// input q by w^q, w = exp(2 pi i k / (NS R)), k = j % NS, and its output q
+// goes to index (j - k) R + k + q NS.
 template <int L, int B, int BS, int R, int NS, int NT>
-__device__ __forceinline__ void stockham_stage(float2* s, const TwTable<L>& T, int tid) {
-    constexpr int LR = L / R;
-    constexpr int NBUT = LR * B;                      // butterflies in the stage
-    constexpr int NB = (NBUT + NT - 1) / NT;          // per thread
-    float2 x[NB][R];
+struct StockhamStage {
+    static constexpr int LR = L / R;
+    static constexpr int NBUT = LR * B;               // butterflies in the stage
+    static constexpr int NB = (NBUT + NT - 1) / NT;   // per thread
+
+    static __device__ __forceinline__ bool active(int beta) {
+        return NBUT % NT == 0 || beta < NBUT;
+    }
+
+    // read + twiddle + radix-R DFT of this thread's butterflies
+    static __device__ __forceinline__ void compute(const float2* s, const TwTable<L>& T,
+                                                   int tid, float2 (&x)[NB][R]) {
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-        const int beta = tid + r * NT;
-        if (NBUT % NT == 0 || beta < NBUT) {
-            const int b = beta % B;
-            const int j = beta / B;
+        for (int r = 0; r < NB; ++r) {
+            const int beta = tid + r * NT;
+            if (active(beta)) {
+                const int b = beta % B;
+                const int j = beta / B;
 #pragma unroll
-            for (int q = 0; q < R; ++q) x[r][q] = s[lds_at<BS>(j + q * LR, b)];
-            if constexpr (NS > 1) {
-                const int k = j % NS;
-                // w = exp(2 pi i k / (NS R)); twiddle of input q is w^q
-                float2 w[R];
-                w[1] = tw_lookup<L>(T, k * (L / (NS * R)));
+                for (int q = 0; q < R; ++q) x[r][q] = s[lds_at<BS>(j + q * LR, b)];
+                if constexpr (NS > 1) {
+                    const int k = j % NS;
+                    float2 w[R];
+                    w[1] = tw_lookup<L>(T, k * (L / (NS * R)));
 #pragma unroll
-                for (int q = 2; q < R; ++q) {
-                    const int h = (q & (q - 1)) ? (1 << ilog2_c(q)) : (q >> 1);
-                    w[q] = cmul(w[h], w[q - h]);
+                    for (int q = 2; q < R; ++q) {
+                        const int h = (q & (q - 1)) ? (1 << ilog2_c(q)) : (q >> 1);
+                        w[q] = cmul(w[h], w[q - h]);
+                    }
+#pragma unroll
+                    for (int q = 1; q < R; ++q) x[r][q] = cmul(x[r][q], w[q]);
                 }
-#pragma unroll
-                for (int q = 1; q < R; ++q) x[r][q] = cmul(x[r][q], w[q]);
+                dft_reg<R>(x[r]);
             }
-            dft_reg<R>(x[r]);
         }
     }
-    __syncthreads();
+
+    // first stage reading its inputs from src(i, b) instead of LDS
+    template <class S>
+    static __device__ __forceinline__ void compute_from(S&& src, int tid,
+                                                        float2 (&x)[NB][R]) {
+        static_assert(NS == 1, "only the first stage reads from a source");
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-        const int beta = tid + r * NT;
-        if (NBUT % NT == 0 || beta < NBUT) {
-            const int b = beta % B;
-            const int j = beta / B;
-            const int k = j % NS;
-            const int base = (j - k) * R + k;
+        for (int r = 0; r < NB; ++r) {
+            const int beta = tid + r * NT;
+            if (active(beta)) {
+                const int b = beta % B;
+                const int j = beta / B;
 #pragma unroll
-            for (int q = 0; q < R; ++q) s[lds_at<BS>(base + q * NS, b)] = x[r][q];
+                for (int q = 0; q < R; ++q) x[r][q] = src(j + q * LR, b);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r)
+            if (active(tid + r * NT)) dft_reg<R>(x[r]);
+    }
+
+    // hand every output (index, sequence, value) to f
+    template <class F>
+    static __device__ __forceinline__ void emit(int tid, const float2 (&x)[NB][R], F&& f) {
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int beta = tid + r * NT;
+            if (active(beta)) {
+                const int b = beta % B;
+                const int j = beta / B;
+                const int k = j % NS;
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int q = 0; q < R; ++q) f(base + q * NS, b, x[r][q]);
+            }
         }
     }
-    __syncthreads();
-}
+
+    // in-place stage: returns synchronised
+    static __device__ __forceinline__ void run(float2* s, const TwTable<L>& T, int tid) {
+        float2 x[NB][R];
+        compute(s, T, tid, x);
+        __syncthreads();
+        emit(tid, x, [&](int i, int b, float2 v) { s[lds_at<BS>(i, b)] = v; });
+        __syncthreads();
+    }
+};
 
 template <int L, int B, int BS, int NS, int NT>
 struct LdsFftStages {
     static __device__ __forceinline__ void run(float2* s, const TwTable<L>& T, int tid) {
         if constexpr (NS < L) {
             constexpr int R = next_radix(L / NS);
-            stockham_stage<L, B, BS, R, NS, NT>(s, T, tid);
+            StockhamStage<L, B, BS, R, NS, NT>::run(s, T, tid);
             LdsFftStages<L, B, BS, NS * R, NT>::run(s, T, tid);
+        }
+    }
+    // all stages in place except the last, whose outputs go to sink(i, b, v)
+    // straight from registers (no LDS write, no barrier after it)
+    template <class F>
+    static __device__ __forceinline__ void run_to(float2* s, const TwTable<L>& T, int tid,
+                                                  F&& sink) {
+        constexpr int R = next_radix(L / NS);
+        using St = StockhamStage<L, B, BS, R, NS, NT>;
+        if constexpr (NS * R < L) {
+            St::run(s, T, tid);
+            LdsFftStages<L, B, BS, NS * R, NT>::run_to(s, T, tid, sink);
+        } else {
+            float2 x[St::NB][R];
+            St::compute(s, T, tid, x);
+            St::emit(tid, x, sink);
         }
     }
 };
@@ -228,6 +286,32 @@ template <int L, int B, int BS, int NT>
 __device__ __forceinline__ void lds_fft(float2* s, const TwTable<L>& T, int tid) {
     static_assert(L >= 2 && L <= 4096 && smooth235(L), "L must be 2^a 3^b 5^c <= 4096");
     LdsFftStages<L, B, BS, 1, NT>::run(s, T, tid);
+}
+
+// The same transform with the first stage's inputs read from src(i, b)
+// (natural order index i of sequence b) rather than from s: the loads of all
+// of a thread's first-stage butterflies are in flight together.  s need not
+// be initialised; T must be complete at the first barrier.  Returns
+// synchronised with the result in s.
+template <int L, int B, int BS, int NT, class S>
+__device__ __forceinline__ void lds_fft_from(float2* s, const TwTable<L>& T, int tid, S&& src) {
+    static_assert(L >= 2 && L <= 4096 && smooth235(L), "L must be 2^a 3^b 5^c <= 4096");
+    constexpr int R = next_radix(L);
+    using St = StockhamStage<L, B, BS, R, 1, NT>;
+    float2 x[St::NB][R];
+    St::compute_from(src, tid, x);
+    St::emit(tid, x, [&](int i, int b, float2 v) { s[lds_at<BS>(i, b)] = v; });
+    __syncthreads();
+    LdsFftStages<L, B, BS, R, NT>::run(s, T, tid);
+}
+
+// The same transform with the last stage's outputs handed to sink(i, b, v)
+// (natural order index i of sequence b) instead of being stored; s is left
+// holding the second-to-last stage.  No barrier at the end.
+template <int L, int B, int BS, int NT, class F>
+__device__ __forceinline__ void lds_fft_to(float2* s, const TwTable<L>& T, int tid, F&& sink) {
+    static_assert(L >= 2 && L <= 4096 && smooth235(L), "L must be 2^a 3^b 5^c <= 4096");
+    LdsFftStages<L, B, BS, 1, NT>::run_to(s, T, tid, sink);
 }
 
 // LDS row stride for B interleaved sequences (bank spreading comes from the

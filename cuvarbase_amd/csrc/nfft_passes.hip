@@ -133,16 +133,15 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     }
     __syncthreads();
 
-    lds_fft<N1, C1, BS, NT1>(tile, T, tid);
-
+    // column FFT; its last stage writes straight from registers to HBM with
+    // the inter-pass twiddle exp(2 pi i u2 k1 / n) (consecutive lanes hold
+    // consecutive k1 of a column, so the stores coalesce)
     float2* out = grids + G.grid + (int64_t)gi * n;
-    for (int i = tid; i < N1 * C1; i += NT1) {
-        const int k1 = i % N1;
-        const int jj = i / N1;
+    lds_fft_to<N1, C1, BS, NT1>(tile, T, tid, [&](int k1, int jj, float2 v) {
         const float2 eh = ehi[jj * NH + (k1 >> 6)];
         const float2 w = (k1 & 63) ? cmul(eh, elo[jj * 64 + (k1 & 63)]) : eh;
-        out[(int64_t)(a + jj) * N1 + k1] = cmul(tile[lds_at<BS>(k1, jj)], w);
-    }
+        out[(int64_t)(a + jj) * N1 + k1] = cmul(v, w);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -177,36 +176,16 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const int n = G.n;
     const int r0 = rb * CR;
 
-    // load the tile: all of a thread's loads are issued before any is
-    // written to LDS (up to LOAD_BATCH in flight per thread)
+    // row FFTs of the tile (rows r0 .. r0+CR-1 of the three grids); the first
+    // stage loads its inputs straight from HBM
     const float2* src = grids + G.grid;
-    constexpr int TOTAL = N2 * B;
-    constexpr int PER = (TOTAL + NT - 1) / NT;
-    constexpr int LOAD_BATCH = PER < 16 ? PER : 16;
-#pragma unroll
-    for (int q0 = 0; q0 < PER; q0 += LOAD_BATCH) {
-        float2 buf[LOAD_BATCH];
-#pragma unroll
-        for (int q = 0; q < LOAD_BATCH; ++q) {
-            const int i = tid + (q0 + q) * NT;
-            const int bb = i % B;
-            const int u2 = i / B;
-            const int gi = bb / CR;
-            const int r = bb - gi * CR;
-            buf[q] = (i < TOTAL && q0 + q < PER && r0 + r < n1)
-                         ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
-                         : make_float2(0.f, 0.f);
-        }
-#pragma unroll
-        for (int q = 0; q < LOAD_BATCH; ++q) {
-            const int i = tid + (q0 + q) * NT;
-            if (i < TOTAL && q0 + q < PER) tile[lds_at<BS>(i / B, i % B)] = buf[q];
-        }
-    }
     build_twiddles<N2, NT>(T, tid);
-    __syncthreads();
-
-    lds_fft<N2, B, BS, NT>(tile, T, tid);
+    lds_fft_from<N2, B, BS, NT>(tile, T, tid, [&](int u2, int bb) -> float2 {
+        const int gi = bb / CR;
+        const int r = bb - gi * CR;
+        return (r0 + r < n1) ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
+                             : make_float2(0.f, 0.f);
+    });
 
     const int K = G.nf;
     const int kc = G.kc;
