@@ -4,7 +4,7 @@
 //   X[k] = sum_u x[u] exp(+2 pi i u k / L),
 // the convention the reference obtains from cuFFT's inverse C2C
 // (cuvarbase/cunfft.py:288, pinned by cuvarbase/tests/test_nfft.py:186-200).
-// Lengths are 2^a 3^b 5^c <= 4096; stages use radices 16, 8, 4, 2, 5, 3.
+// Lengths are 2^a 3^b 5^c <= 4096; stages use radices 16, 12, 9, 8, 4, 2, 5, 3.
 // Stage twiddles come from a two-level table of exp(2 pi i x / L) in LDS
 // (64 + ceil(L/64) entries) and integer powers by complex products.
 #pragma once
@@ -25,10 +25,46 @@ constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v >> 1); }
 constexpr int brev_c(int v, int bits) {
     return bits == 0 ? 0 : ((v & 1) << (bits - 1)) | brev_c(v >> 1, bits - 1);
 }
+// radix of the next Stockham stage for a remaining length rem: the largest
+// in-register DFT that divides it (fewer stages = fewer LDS round trips)
 constexpr int next_radix(int rem) {
-    return rem % 16 == 0 ? 16 : rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2
-         : rem % 5 == 0 ? 5 : rem % 3 == 0 ? 3 : rem;
+    return rem % 16 == 0 ? 16 : rem % 12 == 0 ? 12 : rem % 9 == 0 ? 9 : rem % 8 == 0 ? 8
+         : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 5 == 0 ? 5 : rem % 3 == 0 ? 3 : rem;
 }
+
+// compile-time cos / sin (Taylor series after reduction to [-pi, pi]) for the
+// twiddles inside composite in-register DFTs
+constexpr double kPiC = 3.14159265358979323846;
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i < 24; ++i) {
+        term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int i = 1; i < 24; ++i) {
+        term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+// exp(+2 pi i m / R), m = 0 .. R-1
+template <int R>
+struct RootsOfUnity {
+    float c[R];
+    float s[R];
+    constexpr RootsOfUnity() : c(), s() {
+        for (int m = 0; m < R; ++m) {
+            double x = 2.0 * kPiC * m / R;
+            if (x > kPiC) x -= 2.0 * kPiC;
+            c[m] = (float)taylor_cos(x);
+            s[m] = (float)taylor_sin(x);
+        }
+    }
+};
 constexpr bool smooth235(int v) {
     while (v % 2 == 0) v /= 2;
     while (v % 3 == 0) v /= 3;
@@ -61,10 +97,47 @@ __device__ __forceinline__ float2 mul_w16(float2 v, int e) {
     }
 }
 
+template <int R>
+__device__ __forceinline__ void dft_reg(float2 (&x)[R]);
+
+// composite in-register DFT of length R1 R2 (Cooley-Tukey with n = n1 + R1 n2,
+// k = R2 k1 + k2): length-R2 DFTs over n2, twiddles exp(2 pi i n1 k2 / R),
+// length-R1 DFTs over n1
+template <int R1, int R2>
+__device__ __forceinline__ void dft_ct(float2 (&x)[R1 * R2]) {
+    constexpr int R = R1 * R2;
+    constexpr RootsOfUnity<R> W{};
+    float2 y[R1][R2];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) y[n1][n2] = x[n1 + R1 * n2];
+        dft_reg<R2>(y[n1]);
+#pragma unroll
+        for (int k2 = 1; k2 < R2; ++k2) {
+            const int e = (n1 * k2) % R;
+            if (e != 0) y[n1][k2] = cmul(y[n1][k2], make_float2(W.c[e], W.s[e]));
+        }
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+        float2 z[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) z[n1] = y[n1][k2];
+        dft_reg<R1>(z);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) x[R2 * k1 + k2] = z[k1];
+    }
+}
+
 // In-register DFT: x[r] <- sum_q x[q] exp(+2 pi i q r / R)
 template <int R>
 __device__ __forceinline__ void dft_reg(float2 (&x)[R]) {
-    if constexpr (R == 3) {
+    if constexpr (R == 9) {
+        dft_ct<3, 3>(x);
+    } else if constexpr (R == 12) {
+        dft_ct<4, 3>(x);
+    } else if constexpr (R == 3) {
         constexpr float h = 0.86602540378443864676f;   // sqrt(3)/2
         const float2 t1 = cadd(x[1], x[2]);
         const float2 t2 = csub(x[1], x[2]);

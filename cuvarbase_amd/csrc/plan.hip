@@ -74,7 +74,7 @@ struct Layout {
 
 int radix_stages(int L) {
     int st = 0, rem = L;
-    const int radices[] = {16, 8, 4, 2, 5, 3};
+    const int radices[] = {16, 12, 9, 8, 4, 2, 5, 3};   // the order of next_radix
     while (rem > 1) {
         bool found = false;
         for (int r : radices)
