@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--sigma", type=float, default=None,
                     help="NFFT oversampling (default: tuned for |dP| <= 1e-4)")
     ap.add_argument("--m", type=int, default=None)
+    ap.add_argument("--window", type=int, default=None,
+                    help="NFFT window: 0 Gaussian, 1 exponential of semicircle (default: tuned)")
     ap.add_argument("--stream-requests", type=int, default=1500,
                     help="requests per arrival rate in the serving leg (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
@@ -145,9 +147,14 @@ def main():
 
     from cuvarbase_amd import synthetic, tuning
 
-    sigma, m = tuning.choose_nfft_params(TOL)
-    sigma = args.sigma if args.sigma is not None else sigma
-    m = args.m if args.m is not None else m
+    cfg = tuning.choose_nfft_config(TOL)
+    if args.sigma is not None or args.m is not None or args.window is not None:
+        cfg = dict(sigma=args.sigma if args.sigma is not None else cfg["sigma"],
+                   m=args.m if args.m is not None else cfg["m"],
+                   window=args.window if args.window is not None else cfg["window"],
+                   window_beta=0.0)
+    sigma, m = cfg["sigma"], cfg["m"]
+    win = dict(window=cfg["window"], window_beta=cfg["window_beta"])
     data, freqs = synthetic.survey(args.ncurves)
     total_evals = int(sum(len(f) for f in freqs))
 
@@ -157,7 +164,7 @@ def main():
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
 
     chunk = args.grid_chunk_mib << 20 if args.grid_chunk_mib > 0 else 0
-    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk)
+    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk, **win)
     s = _device.Stream()
     batch.upload(stream=s)
     s.synchronize()
@@ -263,7 +270,7 @@ def main():
 
         arr, sdata, sfreqs = synthetic.stream_requests(1.0, nreq=args.stream_requests)
         srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m,
-                        zero_copy=True)
+                        zero_copy=True, **win)
         serving = {"requests_per_rate": args.stream_requests, "hbm_budget_gb": args.hbm_budget_gb,
                    "rates": {}}
         for rate in (100.0, 1000.0, 10000.0, float("inf")):
@@ -297,7 +304,7 @@ def main():
             del batch
         _device.synchronize()
         shd, shf = synthetic.short(args.short_curves)
-        sb = engine.LombBatch(shd, shf, sigma=sigma, m=m)
+        sb = engine.LombBatch(shd, shf, sigma=sigma, m=m, **win)
         sb.upload(stream=s)
         k = 5
         tp = _device.DeviceArray(sb.ncurves * k, np.float32)
@@ -348,6 +355,8 @@ def main():
             "evals_per_step": total_evals,
             "sigma": sigma,
             "m": m,
+            "window": "exponential of semicircle" if cfg["window"] == 1 else "gaussian",
+            "window_beta": cfg["window_beta"],
             "grid_cells_per_step": int(3 * sum(g[0] for g in geo)),
             "grid_chunk_bytes": int(chunk),
             "workspace_bytes": int(ws_bytes),

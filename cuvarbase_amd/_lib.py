@@ -32,6 +32,8 @@ INPUT_WEIGHTED = 1
 METHOD_NFFT = 0
 METHOD_DIRECT = 1
 METHOD_DIRECT_CUSTOM = 2
+WINDOW_GAUSSIAN = 0
+WINDOW_ES = 1
 
 
 class GLSError(RuntimeError):
@@ -60,6 +62,7 @@ class gls_params(C.Structure):
     _fields_ = [("sigma", C.c_double), ("m", C.c_int32), ("mode", C.c_int32),
                 ("reg_c", C.c_double), ("reg_s", C.c_double),
                 ("input_kind", C.c_int32), ("method", C.c_int32),
+                ("window", C.c_int32), ("window_beta", C.c_float),
                 ("grid_chunk_bytes", C.c_int64)]
 
 

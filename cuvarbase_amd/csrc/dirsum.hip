@@ -149,6 +149,66 @@ __global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __re
     }
 }
 
+// The GLS is ill-conditioned where a frequency spans few cycles over the
+// baseline (CC SS - CS^2 -> 0): there the FP32 round-off of the NFFT sums,
+// largest at the band edge where the lowest frequencies sit, reaches |dP| ~
+// 1e-4.  These few frequencies are recomputed by direct sums with FP64
+// phases and FP64 accumulation (one block per curve, one frequency per
+// thread, points staged through LDS).
+// Thread layout: 16 frequency slots x 16 point lanes; a slot's 16 lanes sum
+// over the points with stride 16 and are reduced by shuffles.
+template <int OUT>
+__global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict__ geom,
+                                                     const CurveState* __restrict__ state,
+                                                     PointArrays pa, PlanParams pp,
+                                                     void* __restrict__ out) {
+    constexpr int LANES = 16;
+    constexpr int SLOTS = NT / LANES;
+    const int tid = threadIdx.x;
+    const int lane = tid % LANES;
+    const int slot = tid / LANES;
+    const CurveGeom& G = geom[blockIdx.x];
+    const CurveState st = state[blockIdx.x];
+    const int N = G.n_obs;
+    // number of leading frequencies with |f| (t_max - t_min) < kLowCycles
+    int nlow = 0;
+    {
+        const int lim = min(G.nf, kLowMax);
+        while (nlow < lim && fabs(G.f0 + (double)nlow * G.df) * st.tspan < kLowCycles) ++nlow;
+    }
+    for (int k0 = 0; k0 < nlow; k0 += SLOTS) {
+        const int k = k0 + slot;
+        const bool active = k < nlow;
+        const double f = G.f0 + (double)k * G.df;
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            for (int j = lane; j < N; j += LANES) {
+                const double x = f * pa.dt[G.pts + j];
+                double sn, cs;
+                sincospi(2.0 * (x - rint(x)), &sn, &cs);
+                const double w = pa.w[G.pts + j], yw = pa.yw[G.pts + j];
+                acc[0] += w * cs;
+                acc[1] += w * sn;
+                acc[2] += w * (cs * cs - sn * sn);
+                acc[3] += w * (2.0 * cs * sn);
+                acc[4] += yw * cs;
+                acc[5] += yw * sn;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+#pragma unroll
+            for (int o = LANES / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, LANES);
+        }
+        if (active && lane == 0) {
+            const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
+                                       st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+            if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+            else ((double*)out)[G.out + k] = P;
+        }
+    }
+}
+
 template <int RF>
 hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
                            const int32_t* tile_curve, const int32_t* tile_first, int ntiles,
@@ -164,6 +224,19 @@ hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
 }
 
 }  // namespace
+
+hipError_t launch_lowfreq(const CurveGeom* d_geom, const CurveState* state, int ncurves,
+                          PointArrays pa, const PlanParams& pp, void* power, int out_f64,
+                          hipStream_t s) {
+    if (ncurves <= 0) return hipSuccess;
+    if (out_f64)
+        hipLaunchKernelGGL(lowfreq_kernel<1>, dim3(ncurves), dim3(NT), 0, s, d_geom, state, pa,
+                           pp, power);
+    else
+        hipLaunchKernelGGL(lowfreq_kernel<0>, dim3(ncurves), dim3(NT), 0, s, d_geom, state, pa,
+                           pp, power);
+    return hipGetLastError();
+}
 
 hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,
                                 const int32_t* tile_curve, const int32_t* tile_first,

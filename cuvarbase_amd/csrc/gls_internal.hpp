@@ -86,7 +86,14 @@ struct PlanParams {
     int32_t input_kind;
     int32_t method;
     int32_t kind;      // 0 = lomb, 1 = nfft
+    int32_t window;    // GLS_WINDOW_*
+    float es_beta;     // ES shape parameter
+    // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (device copy
+    // in the workspace; set per execute)
+    const float* es_deconv;
 };
+
+constexpr int kEsTab = 16384;   // ES deconvolution table intervals over xi in [0, 1/2]
 
 // pass-1 columns per tile: power of two with n1 * c1 <= 4096, at most 16
 constexpr int pass1_cols(int n1) {
@@ -111,7 +118,8 @@ hipError_t launch_prep(const CurveGeom* d_geom, int ncurves, CurveState* state,
                        int y_is_f32, hipStream_t s);
 hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves,
                         int ncurves, int ngrids, int p2, PointArrays pa,
-                        const int32_t* bkt, float2* grids, int m, hipStream_t s);
+                        const int32_t* bkt, float2* grids, const PlanParams& pp,
+                        hipStream_t s);
 hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom,
                              const CurveState* state, const int32_t* curves,
                              int ncurves, const float2* grids,
@@ -119,13 +127,20 @@ hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom,
                              hipStream_t s);
 hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom,
                              const CurveState* state, const int32_t* curves,
-                             int ncurves, const float2* grids, float2* out,
-                             hipStream_t s);
+                             int ncurves, const float2* grids, const PlanParams& pp,
+                             float2* out, hipStream_t s);
 hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const int32_t* tile_curve, const int32_t* tile_first,
                          int ntiles, int rf, PointArrays pa,
                          const PlanParams& pp, void* power, int out_f64,
                          hipStream_t s);
+// direct FP64 sums for a curve's lowest frequencies, f_k (t_max - t_min) <
+// kLowCycles (at most kLowMax of them), overwriting the NFFT powers there
+constexpr double kLowCycles = 3.0;
+constexpr int kLowMax = 256;
+hipError_t launch_lowfreq(const CurveGeom* d_geom, const CurveState* state, int ncurves,
+                          PointArrays pa, const PlanParams& pp, void* power, int out_f64,
+                          hipStream_t s);
 hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,
                                 const int32_t* tile_curve, const int32_t* tile_first,
                                 int ntiles, PointArrays pa, const PlanParams& pp,

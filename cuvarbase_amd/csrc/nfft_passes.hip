@@ -31,7 +31,8 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
                                                     const int32_t* __restrict__ curves,
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
-                                                    float2* __restrict__ grids, int m) {
+                                                    float2* __restrict__ grids, int m,
+                                                    int window, float es_beta) {
     constexpr int C1 = pass1_cols(N1);
     constexpr int BS = lds_stride(C1);
     constexpr int NH = (N1 + 63) / 64;
@@ -104,8 +105,14 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     }
     __syncthreads();
 
+    // window weight of a point at distance d (cells) from a cell:
+    // Gaussian exp(-d^2/b)/sqrt(pi b), or ES exp(beta (sqrt(1 - z^2) - 1)),
+    // z = d/(m + 1/2), zero for |z| >= 1
+    const bool es = window == GLS_WINDOW_ES;
     const float binv = (float)(1.4426950408889634 / G.b);   // log2(e)/b
     const float norm = (float)(1.0 / sqrt(M_PI * G.b));
+    const float ainv = 1.0f / ((float)m + 0.5f);
+    const float bl2 = es_beta * 1.4426950408889634f;          // beta log2(e)
     const int two_m = 2 * m;
     auto spread = [&](int U, float fr, float2 v) {
         const int col0 = U & (n2 - 1);
@@ -117,7 +124,14 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
                 cell = cell < 0 ? cell + n : (cell >= n ? cell - n : cell);
                 const int u1 = cell >> p2;
                 const float d = (float)(m - k) + fr;
-                const float wgt = exp2f(-d * d * binv) * norm;
+                float wgt;
+                if (es) {
+                    const float z = d * ainv;
+                    const float q = 1.0f - z * z;
+                    wgt = q > 0.f ? exp2f(bl2 * (sqrtf(q) - 1.0f)) : 0.f;
+                } else {
+                    wgt = exp2f(-d * d * binv) * norm;
+                }
                 float2* cellp = &tile[lds_at<BS>(u1, jj)];
                 atomicAdd(&cellp->x, v.x * wgt);
                 atomicAdd(&cellp->y, v.y * wgt);
@@ -202,7 +216,17 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
         else if (l >= n - kc) lp = l - n;
         else continue;
         const float xi = (float)lp * ninv;
-        const float dc = expf(bpi2 * xi * xi);
+        float dc;
+        if (pp.window == GLS_WINDOW_ES) {
+            // 1/psi_hat(|xi|) by linear interpolation in the plan's table
+            const float u = fabsf(xi) * (2.0f * kEsTab);
+            const int i0 = min((int)u, kEsTab);
+            const float f = u - (float)i0;
+            const float t0 = pp.es_deconv[i0];
+            dc = t0 + f * (pp.es_deconv[i0 + 1] - t0);
+        } else {
+            dc = expf(bpi2 * xi * xi);
+        }
         const int k = lp + kc;
         if (OUT == 2) {
             const float2 V = cscale(tile[lds_at<BS>(k2, r)], dc);
@@ -226,13 +250,13 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
 template <int N1>
 hipError_t pass1_dispatch(const CurveGeom* d_geom, const int32_t* curves, int ncurves,
                           int ngrids, int p2, PointArrays pa, const int32_t* bkt,
-                          float2* grids, int m, hipStream_t s) {
+                          float2* grids, const PlanParams& pp, hipStream_t s) {
     constexpr int C1 = pass1_cols(N1);
     const int nb = (1 << p2) / C1;
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, d_geom,
-                       curves, ngrids, p2, pa, bkt, grids, m);
+                       curves, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta);
     return hipGetLastError();
 }
 
@@ -269,11 +293,11 @@ hipError_t pass2_switch(int p2, int n1, const CurveGeom* d_geom, const CurveStat
 
 hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves, int ncurves,
                         int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
-                        int m, hipStream_t s) {
+                        const PlanParams& pp, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
     switch (n1) {
 #define GLS_PASS1_CASE(v) \
-    case v: return pass1_dispatch<v>(d_geom, curves, ncurves, ngrids, p2, pa, bkt, grids, m, s);
+    case v: return pass1_dispatch<v>(d_geom, curves, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
         GLS_PASS1_SIZES(GLS_PASS1_CASE)
 #undef GLS_PASS1_CASE
         default: return hipErrorInvalidValue;
@@ -291,9 +315,8 @@ hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom, const Curv
 
 hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
                              const int32_t* curves, int ncurves, const float2* grids,
-                             float2* out, hipStream_t s) {
+                             const PlanParams& pp, float2* out, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
-    PlanParams pp{};
     return pass2_switch<2>(p2, n1, d_geom, state, curves, ncurves, grids, pp, out, s);
 }
 

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "gls_internal.hpp"
 
@@ -41,7 +42,8 @@ struct gls_plan {
     int rf = 1;
     // host image of the descriptors uploaded into workspace[0 .. meta.size())
     std::vector<char> meta;
-    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0;
+    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0, off_estab = 0;
+    std::vector<float> estab;      // ES deconvolution table (empty for the Gaussian)
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
@@ -189,6 +191,10 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         P->off_U2 = L.take(sizeof(int32_t) * total_pts);
         P->off_fr2 = L.take(sizeof(float) * total_pts);
         P->off_v2 = L.take(sizeof(float2) * total_pts);
+        // prepared points for the low-frequency direct sums
+        P->off_w = L.take(sizeof(float) * total_pts);
+        P->off_yw = L.take(sizeof(float) * total_pts);
+        P->off_dt = L.take(sizeof(double) * total_pts);
     }
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
@@ -241,6 +247,61 @@ int plan_direct_tiles(gls_plan* P, std::vector<int32_t>& tiles) {
     return GLS_OK;
 }
 
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n)
+void gauss_legendre(int nq, std::vector<double>& x, std::vector<double>& w) {
+    x.resize(nq);
+    w.resize(nq);
+    for (int i = 0; i < (nq + 1) / 2; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (nq + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = 0.0;
+            for (int j = 1; j <= nq; ++j) {
+                const double p2 = p1;
+                p1 = p0;
+                p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p2) / j;
+            }
+            dp = nq * (z * p0 - p1) / (z * z - 1.0);
+            const double dz = p0 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-15) break;
+        }
+        x[i] = -z;
+        x[nq - 1 - i] = z;
+        w[i] = w[nq - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+// 1/psi_hat(xi) of the ES window psi(x) = exp(beta (sqrt(1 - (x/a)^2) - 1)),
+// a = m + 1/2 cells, at xi = i/(2 kEsTab), i = 0 .. kEsTab + 1:
+// psi_hat(xi) = 2 int_0^a psi(x) cos(2 pi xi x) dx by Gauss-Legendre.
+// Cached per (m, beta): plans are created per batch.
+const std::vector<float>& es_deconv_table(int m, float beta) {
+    static std::mutex mu;
+    static std::map<std::pair<int, float>, std::vector<float>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({m, beta});
+    if (it != cache.end()) return it->second;
+    const int nq = 200;
+    std::vector<double> gx, gw;
+    gauss_legendre(nq, gx, gw);
+    const double a = m + 0.5;
+    std::vector<double> xs(nq), ws(nq);
+    for (int q = 0; q < nq; ++q) {   // map [-1, 1] -> [0, a]
+        const double x = 0.5 * a * (gx[q] + 1.0);
+        const double z = x / a;
+        xs[q] = x;
+        ws[q] = 0.5 * a * gw[q] * std::exp((double)beta * (std::sqrt(std::max(0.0, 1.0 - z * z)) - 1.0));
+    }
+    std::vector<float> tab(kEsTab + 2);
+    for (int i = 0; i <= kEsTab + 1; ++i) {
+        const double xi = (double)i / (2.0 * kEsTab);
+        double acc = 0.0;
+        for (int q = 0; q < nq; ++q) acc += ws[q] * std::cos(2.0 * M_PI * xi * xs[q]);
+        tab[i] = (float)(1.0 / (2.0 * acc));
+    }
+    return cache.emplace(std::make_pair(m, beta), std::move(tab)).first->second;
+}
+
 // Algorithmic bytes per stage: what each kernel must read and write at
 // minimum for its role in the fused pipeline.
 void build_stage_list(gls_plan* P) {
@@ -265,6 +326,11 @@ void build_stage_list(gls_plan* P) {
             P->stages.push_back({1, B.n1 << B.p2, b1});
             P->stages.push_back({2, B.n1 << B.p2, b2});
         }
+        if (P->kind == 0) {
+            double bl = 0.0;
+            for (auto& G : P->geom) bl += 16.0 * G.n_obs;
+            P->stages.push_back({4, 0, bl});
+        }
     } else {
         double b = 0.0;
         for (auto& G : P->geom) b += 16.0 * G.n_obs + 4.0 * G.nf;
@@ -276,12 +342,16 @@ void build_stage_list(gls_plan* P) {
 void reserve_meta(gls_plan* P, Layout& L, size_t nbcurves, size_t ntiles_ints) {
     P->off_geom = L.take(sizeof(CurveGeom) * P->geom.size());
     P->off_bcurves = L.take(sizeof(int32_t) * nbcurves);
+    P->off_estab = L.take(sizeof(float) * P->estab.size());
     P->off_tiles = L.take(sizeof(int32_t) * ntiles_ints);
 }
 
 void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
                const std::vector<int32_t>& tiles) {
     P->meta.assign(P->off_tiles + sizeof(int32_t) * tiles.size(), 0);
+    if (!P->estab.empty())
+        std::memcpy(P->meta.data() + P->off_estab, P->estab.data(),
+                    sizeof(float) * P->estab.size());
     if (!P->geom.empty())
         std::memcpy(P->meta.data() + P->off_geom, P->geom.data(),
                     sizeof(CurveGeom) * P->geom.size());
@@ -309,9 +379,10 @@ PointArrays point_arrays(const gls_plan* P, void* ws) {
     pa.U2 = (int32_t*)(base + P->off_U2);
     pa.fr2 = (float*)(base + P->off_fr2);
     pa.v2 = (float2*)(base + P->off_v2);
-    pa.w = (float*)(base + P->off_w);
-    pa.yw = (float*)(base + P->off_yw);
-    pa.dt = (double*)(base + P->off_dt);
+    // offset 0 is the descriptor region: it marks an array the plan lacks
+    pa.w = P->off_w ? (float*)(base + P->off_w) : nullptr;
+    pa.yw = P->off_yw ? (float*)(base + P->off_yw) : nullptr;
+    pa.dt = P->off_dt ? (double*)(base + P->off_dt) : nullptr;
     return pa;
 }
 
@@ -335,6 +406,8 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         return hipEventRecord(P->events[ev++], s);
     };
     const int ngrids = (P->kind == 1) ? 1 : 3;
+    PlanParams pp = P->pp;
+    pp.es_deconv = P->estab.empty() ? nullptr : (const float*)(base + P->off_estab);
     GLS_HIP_TRY(mark());
     GLS_HIP_TRY(launch_prep(d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, y_is_f32, s));
     GLS_HIP_TRY(mark());
@@ -361,14 +434,14 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
             const int32_t* curves = d_bcurves + B.first;
             GLS_HIP_TRY(launch_pass1(B.n1, d_geom, curves, B.count, ngrids, B.p2, pa, bkt, grids,
-                                     P->pp.m, ls));
+                                     pp, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
                 GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              (float2*)out, ls));
+                                              pp, (float2*)out, ls));
             else
                 GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              P->pp, out, out_f64, ls));
+                                              pp, out, out_f64, ls));
             GLS_HIP_TRY(mark());
         }
         if (fork) {
@@ -376,6 +449,10 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
                 GLS_HIP_TRY(hipEventRecord(P->fork_join[j], lanes[j]));
                 GLS_HIP_TRY(hipStreamWaitEvent(s, P->fork_join[j], 0));
             }
+        }
+        if (P->kind == 0) {
+            GLS_HIP_TRY(launch_lowfreq(d_geom, state, P->ncurves, pa, pp, out, out_f64, s));
+            GLS_HIP_TRY(mark());
         }
     } else if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) {
         GLS_HIP_TRY(launch_dirsum_custom(d_geom, state, d_tiles, d_tiles + P->ntiles, P->ntiles,
@@ -443,6 +520,10 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     }
     if (!std::isfinite(prm.reg_c) || !std::isfinite(prm.reg_s) || prm.reg_c < 0 || prm.reg_s < 0)
         return fail(GLS_E_INVALID, "regularisation must be finite and >= 0");
+    if (prm.window != GLS_WINDOW_GAUSSIAN && prm.window != GLS_WINDOW_ES)
+        return fail(GLS_E_INVALID, "unknown window");
+    if (!std::isfinite(prm.window_beta) || prm.window_beta > 200.f)
+        return fail(GLS_E_INVALID, "window_beta must be finite and <= 200");
     gls_plan* P = new gls_plan();
     P->kind = 0;
     P->pp.m = prm.m;
@@ -452,6 +533,11 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.input_kind = prm.input_kind;
     P->pp.method = prm.method;
     P->pp.kind = 0;
+    P->pp.window = (prm.method == GLS_METHOD_NFFT) ? prm.window : GLS_WINDOW_GAUSSIAN;
+    P->pp.es_beta = prm.window_beta > 0.f
+                        ? prm.window_beta
+                        : (float)(M_PI * (2 * prm.m + 1) * (1.0 - 0.5 / prm.sigma));
+    if (P->pp.window == GLS_WINDOW_ES) P->estab = es_deconv_table(prm.m, P->pp.es_beta);
     P->sigma = prm.sigma;
     P->chunk_bytes = std::max<int64_t>(prm.grid_chunk_bytes, 0);
     P->ncurves = ncurves;
@@ -500,7 +586,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         // re-take the data regions after the meta and state regions
         const size_t shift = L.cur;
         for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_v1, &P->off_U2,
-                          &P->off_fr2, &P->off_v2, &P->off_bkt, &P->off_grid})
+                          &P->off_fr2, &P->off_v2, &P->off_w, &P->off_yw, &P->off_dt,
+                          &P->off_bkt, &P->off_grid})
             *o += shift;
         L.cur += probe.cur;
     } else {

@@ -132,6 +132,19 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
         return;
     }
 
+    // prepared points for the low-frequency direct sums of lomb NFFT plans
+    if (KIND != 2 && pa.dt != nullptr) {
+        for (int j = tid; j < N; j += NT) {
+            PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
+            double w, yw;
+            weights_of(p, w, yw);
+            const double dt = p.t - tref;
+            pa.w[G.pts + j] = isfinite(dt) ? (float)w : 0.f;
+            pa.yw[G.pts + j] = isfinite(dt) ? (float)yw : 0.f;
+            pa.dt[G.pts + j] = isfinite(dt) ? dt : 0.0;
+        }
+    }
+
     // ---- phase B: cells, modulation and bucketing -------------------------
     const int n = G.n;
     const int p2 = G.p2;

@@ -33,7 +33,7 @@ class Plan:
 
     @classmethod
     def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method,
-             grid_chunk_bytes=0):
+             grid_chunk_bytes=0, window=0, window_beta=0.0):
         _lib.init(0)
         n = len(curves)
         arr = (_lib.gls_curve * max(n, 1))()
@@ -42,6 +42,7 @@ class Plan:
         prm = _lib.gls_params(sigma=float(sigma), m=int(m), mode=int(mode),
                               reg_c=float(reg_c), reg_s=float(reg_s),
                               input_kind=int(input_kind), method=int(method),
+                              window=int(window), window_beta=float(window_beta),
                               grid_chunk_bytes=int(grid_chunk_bytes))
         h = C.c_void_p()
         _lib.call("gls_lomb_plan_create", C.byref(h), arr, C.c_int32(n), C.byref(prm))
@@ -89,7 +90,7 @@ class Plan:
     def enable_timing(self, enable=True):
         _lib.call("gls_plan_enable_timing", self.handle, C.c_int32(1 if enable else 0))
 
-    STAGE_NAMES = {0: "prep", 1: "pass1", 2: "pass2", 3: "dirsum"}
+    STAGE_NAMES = {0: "prep", 1: "pass1", 2: "pass2", 3: "dirsum", 4: "lowfreq"}
 
     def stage_times(self):
         """[(stage, grid n, ms, algorithmic bytes)] of the last timed execute
@@ -128,12 +129,15 @@ class LombBatch:
         direct sums at exactly those values
     grid_chunk_bytes : if > 0, NFFT grids run in chunks of this many bytes
         that reuse one workspace region (smaller workspace, slightly slower)
+    window, window_beta : spreading window of the NFFT path: the reference's
+        Gaussian (``WINDOW_GAUSSIAN``) or the exponential of semicircle
+        (``WINDOW_ES``, shape ``window_beta``; 0 picks the default)
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
                  input_kind="raw", ybar=None, yy=None, custom_freqs=False,
-                 grid_chunk_bytes=0):
+                 grid_chunk_bytes=0, window=_lib.WINDOW_GAUSSIAN, window_beta=0.0):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)
@@ -173,7 +177,7 @@ class LombBatch:
             method = _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT
             self._freqs = None
         self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method,
-                              grid_chunk_bytes)
+                              grid_chunk_bytes, window, window_beta)
         # host staging of the inputs
         t = np.empty(obs, dtype=np.float64)
         if self.input_kind == _lib.INPUT_RAW:

@@ -31,7 +31,7 @@ import numpy as np
 from . import _lib
 from ._device import DeviceArray, Event, Stream, pinned_empty, _handle
 from .engine import Plan, linear_grid
-from .tuning import choose_nfft_params
+from .tuning import choose_nfft_config
 from . import wire
 
 
@@ -221,7 +221,8 @@ class GLSServer:
     hbm_budget : bytes of device memory the server may use (resident arena)
     slots : number of batches that may be in flight at once (one stream each)
     max_batch_evals : cap on curve x frequency evaluations per batch
-    sigma, m : NFFT parameters (default: tuned for |dP| <= ``tol``)
+    sigma, m, window, window_beta : NFFT parameters (default: tuned for
+        |dP| <= ``tol``)
     zero_copy : results are views into a page-locked result arena of
         ``result_buffer_bytes`` (default 4 B x max_batch_evals x (slots + 1))
         that the D2H copies write directly; each request's memory returns to
@@ -231,11 +232,18 @@ class GLSServer:
 
     def __init__(self, hbm_budget=16e9, slots=3, max_batch_evals=3e8, max_batch_curves=8192,
                  sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN, zero_copy=False,
-                 result_buffer_bytes=None):
+                 result_buffer_bytes=None, window=None, window_beta=0.0):
         _lib.init(0)
-        s0, m0 = choose_nfft_params(tol)
-        self.sigma = float(sigma if sigma is not None else s0)
-        self.m = int(m if m is not None else m0)
+        cfg = choose_nfft_config(tol)
+        if sigma is not None or m is not None or window is not None:
+            cfg = dict(sigma=sigma if sigma is not None else cfg["sigma"],
+                       m=m if m is not None else cfg["m"],
+                       window=window if window is not None else cfg["window"],
+                       window_beta=window_beta)
+        self.sigma = float(cfg["sigma"])
+        self.m = int(cfg["m"])
+        self.window = int(cfg["window"])
+        self.window_beta = float(cfg["window_beta"])
         self.mode = mode
         self.max_batch_evals = int(max_batch_evals)
         self.max_batch_curves = int(max_batch_curves)
@@ -327,7 +335,8 @@ class GLSServer:
             obs += len(r.t)
             out += r.nf
         plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
-                         _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT)
+                         _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT,
+                         window=self.window, window_beta=self.window_beta)
         k = max(r.k for r in reqs) if topk else 0
         nreq = len(reqs)
         sizes = [24 * max(obs, 1), 4 * max(out, 1), plan.workspace_bytes]

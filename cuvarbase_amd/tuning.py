@@ -73,6 +73,50 @@ def choose_nfft_params(tol=1e-4, m_max=16):
     return c[0] if c else (2.0, 8)
 
 
+# Exponential-of-semicircle window psi(z) = exp(beta (sqrt(1 - z^2) - 1)) on
+# 2m + 1 cells, beta = gamma pi (2m + 1)(1 - 1/(2 sigma)).  Its transform is
+# flat across the band, so the deconvolution amplifies FP32 round-off far
+# less than the Gaussian's exp(b pi^2 xi^2): the same accuracy needs a smaller
+# sigma and m.  Entries (sigma, m): (gamma, max |dP|) measured with the FP32
+# emulation described above on survey curves, including the hardest ones
+# (50-100 points, whose lowest frequencies are ill-conditioned and are taken
+# by the FP64 direct-sum fix-up of the library: f (t_max - t_min) < 3).
+ES_MEASURED = {
+    (1.25, 6): (1.05, 3.6e-5),
+    (1.3, 5): (1.00, 2.3e-5),
+    (1.3, 6): (1.00, 1.06e-5),
+    (1.35, 5): (1.05, 8.9e-6),
+}
+# relative cost per evaluation: grid cells (sigma) plus spreading taps
+SPREAD_COST_PER_TAP = 0.015
+
+
+def es_beta(sigma, m, gamma=1.0):
+    return gamma * math.pi * (2 * m + 1) * (1.0 - 1.0 / (2.0 * sigma))
+
+
+def config_cost(sigma, m):
+    return sigma * (1.0 + SPREAD_COST_PER_TAP * (2 * m + 1))
+
+
+def choose_nfft_config(tol=1e-4):
+    """Cheapest NFFT configuration for |dP| <= tol / SAFETY among the Gaussian
+    model candidates and the measured ES entries: a dict with sigma, m,
+    window (0 Gaussian, 1 ES) and window_beta."""
+    best = None
+    for sigma, m in candidates(tol):
+        c = config_cost(sigma, m)
+        if best is None or c < best[0]:
+            best = (c, dict(sigma=sigma, m=m, window=0, window_beta=0.0))
+    for (sigma, m), (gamma, err) in ES_MEASURED.items():
+        if err * SAFETY <= tol:
+            c = config_cost(sigma, m)
+            if best is None or c < best[0]:
+                best = (c, dict(sigma=sigma, m=m, window=1,
+                                window_beta=es_beta(sigma, m, gamma)))
+    return best[1] if best else dict(sigma=2.0, m=8, window=0, window_beta=0.0)
+
+
 def calibrate(data, freqs, tol=1e-4, probes=4, nprobe_freqs=256, seed=0):
     """Measured choice on the GPU: returns (sigma, m, measured max |dP|).
 

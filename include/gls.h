@@ -51,6 +51,10 @@ extern "C" {
 #define GLS_INPUT_WEIGHTED  1
 
 /* ---- methods ------------------------------------------------------------ */
+#define GLS_WINDOW_GAUSSIAN 0
+#define GLS_WINDOW_ES       1   /* exponential of semicircle: same accuracy at a smaller
+                                   sigma and m in FP32; no reference counterpart */
+
 #define GLS_METHOD_NFFT     0   /* use_fft=True  (cuvarbase/lombscargle.py:660-684) */
 #define GLS_METHOD_DIRECT   1   /* use_fft=False (cuvarbase/lombscargle.py:643-659) */
 /* direct sums at explicit, arbitrarily spaced frequencies: the reference's
@@ -84,6 +88,9 @@ typedef struct gls_params {
                          /* (cuvarbase/lombscargle.py:103-118)                              */
     int32_t input_kind;  /* GLS_INPUT_*                                                     */
     int32_t method;      /* GLS_METHOD_*                                                    */
+    int32_t window;      /* GLS_WINDOW_GAUSSIAN (the reference's, cuvarbase/cunfft.py:56-57)
+                            or GLS_WINDOW_ES (exp(beta (sqrt(1 - z^2) - 1)) on 2m+1 cells) */
+    float   window_beta; /* ES shape; <= 0: pi (2m+1)(1 - 1/(2 sigma))                  */
     int64_t grid_chunk_bytes; /* > 0: the NFFT grids of a size bucket run in chunks of at
                            most this many bytes that reuse one workspace region (workspace
                            shrinks from all grids of the batch to the largest chunk, at a

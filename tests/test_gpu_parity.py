@@ -410,3 +410,28 @@ def test_multiharmonic_drop_in():
     np.testing.assert_allclose(p, q, atol=1e-6)
     with pytest.raises(NotImplementedError):
         proc.run([(t, y, dy)], freqs=freqs, use_fft=False)
+
+
+def test_es_window_against_oracle():
+    """The tuned exponential-of-semicircle configuration (smaller sigma and m
+    than the Gaussian needs in FP32) against the FP64 oracle NFFT."""
+    from cuvarbase_amd import engine, synthetic, tuning
+
+    cfg = tuning.choose_nfft_config(TOL)
+    data, freqs = synthetic.survey(40, seed=24)
+    P = engine.LombBatch(data, freqs, sigma=cfg["sigma"], m=cfg["m"], window=1,
+                         window_beta=cfg["window_beta"]).run()
+    G = engine.LombBatch(data, freqs, sigma=2.0, m=8).run()
+    worst, same = 0.0, 0
+    for i in range(0, 40, 5):
+        t, y, dy = data[i]
+        Po = on.gls_nfft(t, y, dy, freqs[i], sigma=4.0, m=12)
+        worst = max(worst, float(np.max(np.abs(P[i].astype(float) - Po))))
+        same += int(np.argmax(P[i]) == np.argmax(Po))
+    gap = max(float(np.max(np.abs(p.astype(float) - g))) for p, g in zip(P, G))
+    print(f"ES sigma={cfg['sigma']} m={cfg['m']}: max|dP| {worst:.2e} vs FP64, "
+          f"{gap:.2e} vs Gaussian sigma=2")
+    assert worst < TOL and same == 8
+    # default (beta <= 0) picks a usable shape too
+    d0 = engine.LombBatch(data[:3], freqs[:3], sigma=cfg["sigma"], m=cfg["m"], window=1).run()
+    assert all(np.max(np.abs(a - b)) < TOL for a, b in zip(d0, P[:3]))

@@ -1,10 +1,10 @@
 """Continuous-batching server on the GPU.
 
 A request's periodogram served in a mix and computed alone differ only by the
-order of FP32 additions in the LDS spreading; the deconvolution multiplies
-that round-off by up to exp(b pi^2 / (4 sigma^2)) (~100 at the tuned
-sigma = 1.45, m = 8), so the two agree to MIX_TOL = 2e-5, well inside the
-1e-4 parity bar against FP64.
+order of FP32 additions in the LDS spreading, which the deconvolution
+amplifies at the band edges (up to ~100x for the Gaussian at sigma = 1.45,
+m = 8; less for the exponential-of-semicircle window), so the two agree to
+MIX_TOL = 2e-5, well inside the 1e-4 parity bar against FP64.
 """
 import numpy as np
 import pytest
@@ -31,7 +31,8 @@ def test_each_result_same_alone_and_in_the_mix():
     worst = 0.0
     for r, d, f in zip(st["requests_obj"], data, freqs):
         assert r.done and r.latency > 0
-        alone = engine.LombBatch([d], [f], sigma=srv.sigma, m=srv.m).run()[0]
+        alone = engine.LombBatch([d], [f], sigma=srv.sigma, m=srv.m, window=srv.window,
+                                 window_beta=srv.window_beta).run()[0]
         worst = max(worst, float(np.max(np.abs(r.power.astype(float) - alone))))
     print(f"served {st['requests']} requests in {st['batches']} batches, p50 {st['p50_ms']:.1f} "
           f"ms, p99 {st['p99_ms']:.1f} ms; alone vs mix max|dP| = {worst:.2e}")
@@ -112,7 +113,8 @@ def test_serve_records_mixed_classes():
             np.testing.assert_allclose(out.power, alone, atol=1e-6)
             continue
         alone = engine.LombBatch([(r.t, r.y, r.dy)], [r.frequencies()], sigma=srv.sigma,
-                                 m=srv.m).run()[0]
+                                 m=srv.m, window=srv.window,
+                                 window_beta=srv.window_beta).run()[0]
         if r.k == 0:
             assert np.max(np.abs(out.power - alone)) < MIX_TOL
         else:
