@@ -15,7 +15,16 @@ from oracle import gls as og
 from oracle import nfft as on
 
 
-def _nfft_fp32(t, v, f0, df, K, n, m, b, tref):
+def _es_inv_hat(xi, m, beta, nq=200):
+    """1/psi_hat of the ES window on 2m+1 cells by Gauss-Legendre quadrature."""
+    a = m + 0.5
+    x, w = np.polynomial.legendre.leggauss(nq)
+    x, w = x * a, w * a
+    psi = np.exp(beta * (np.sqrt(np.maximum(1 - (x / a) ** 2, 0)) - 1))
+    return 1.0 / (np.cos(2 * np.pi * np.outer(xi, x)) @ (w * psi))
+
+
+def _nfft_fp32(t, v, f0, df, K, n, m, b, tref, es_beta=None):
     kc = K // 2
     dt = t - tref
     s = on._frac(dt * df)
@@ -27,26 +36,40 @@ def _nfft_fp32(t, v, f0, df, K, n, m, b, tref):
     norm = np.float32(1.0 / np.sqrt(np.pi * b))
     for k in range(2 * m + 1):
         d = np.float32(m - k) + fr
-        wgt = (np.exp(-d * d / np.float32(b)) * norm).astype(np.float32)
+        if es_beta is None:
+            wgt = (np.exp(-d * d / np.float32(b)) * norm).astype(np.float32)
+        else:
+            q = 1 - (d / np.float32(m + 0.5)) ** 2
+            wgt = np.where(q > 0, np.exp(np.float32(es_beta) * (np.sqrt(np.maximum(q, 0)) - 1)),
+                           0).astype(np.float32)
         np.add.at(g, (U + k) % n, (vv * wgt).astype(np.complex64))
     G = sf.ifft(g) * np.float32(n)
     assert G.dtype == np.complex64
     lp = np.arange(K) - kc
-    dc = np.exp(b * (np.pi * lp / n) ** 2).astype(np.float32)
+    if es_beta is None:
+        dc = np.exp(b * (np.pi * lp / n) ** 2).astype(np.float32)
+    else:
+        dc = _es_inv_hat(lp / n, m, es_beta).astype(np.float32)
     return (G[lp % n] * dc).astype(np.complex128)
 
 
-def _gls_fp32(t, y, dy, freqs, sigma, m):
+def _gls_fp32(t, y, dy, freqs, sigma, m, es_beta=None, lowfix=False):
+    t0, y0, dy0 = t, y, dy
     t, y, dy = og.normalize_light_curve(t, y, dy)
     w, yw, ybar, YY = og.prepare(y, dy)
     f0, df, K = freqs[0], freqs[1] - freqs[0], len(freqs)
     n = int(np.ceil(sigma * K))
     b = on.gauss_b(n / K, m)
     tref = t.min()
-    a = _nfft_fp32(t, yw, f0, df, K, n, m, b, tref)
-    c = _nfft_fp32(t, w, f0, df, K, n, m, b, tref)
-    d = _nfft_fp32(t, w, 2 * f0, 2 * df, K, n, m, b, tref)
-    return og.power_from_sums(c.real, c.imag, d.real, d.imag, a.real, a.imag, YY, ybar)
+    a = _nfft_fp32(t, yw, f0, df, K, n, m, b, tref, es_beta)
+    c = _nfft_fp32(t, w, f0, df, K, n, m, b, tref, es_beta)
+    d = _nfft_fp32(t, w, 2 * f0, 2 * df, K, n, m, b, tref, es_beta)
+    P = og.power_from_sums(c.real, c.imag, d.real, d.imag, a.real, a.imag, YY, ybar)
+    if lowfix:
+        # the library's FP64 direct sums at frequencies with f (t_max - t_min) < 3
+        low = np.flatnonzero(np.abs(freqs) * np.ptp(t) < 3.0)[:256]
+        P[low] = og.gls_direct(t0, y0, dy0, np.asarray(freqs)[low])
+    return P
 
 
 @pytest.fixture(scope="module")
@@ -92,3 +115,30 @@ def test_model_tracks_emulation(curves):
         got = float(np.max(np.abs(_gls_fp32(*d, f, sigma, m) - ref)))
         pred = tuning.predicted_power_error(sigma, m)
         assert got < 2 * pred and pred < 30 * max(got, 1e-8), (sigma, m, got, pred)
+
+
+@pytest.fixture(scope="module")
+def hard_curves():
+    """Survey curves with the fewest points: their lowest frequencies are the
+    worst-conditioned GLS problems of the workload."""
+    from cuvarbase_amd import synthetic
+
+    data, freqs = synthetic.survey(40, seed=24)
+    sel = list(np.argsort([len(d[0]) for d in data])[:2])
+    return [(data[i], freqs[i], on.gls_nfft(*data[i], freqs[i], sigma=4.0, m=12)) for i in sel]
+
+
+def test_es_choice_meets_target_with_low_frequency_fixup(hard_curves):
+    from cuvarbase_amd import tuning
+
+    cfg = tuning.choose_nfft_config(1e-4)
+    assert cfg["window"] == 1
+    worst = max(float(np.max(np.abs(_gls_fp32(*d, f, cfg["sigma"], cfg["m"],
+                                              es_beta=cfg["window_beta"], lowfix=True) - ref)))
+                for d, f, ref in hard_curves)
+    assert worst < 2e-5
+    # without the fix-up the ill-conditioned lowest frequencies miss the target
+    raw = max(float(np.max(np.abs(_gls_fp32(*d, f, cfg["sigma"], cfg["m"],
+                                            es_beta=cfg["window_beta"]) - ref)))
+              for d, f, ref in hard_curves)
+    assert raw > worst
