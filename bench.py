@@ -173,8 +173,7 @@ def main():
     s.synchronize()
     copy_gbs = measured_copy_gbs(s, mib=max(args.calib_copy_mib, 256))
 
-    # timed steps: the production launch order (size buckets forked over the
-    # library's streams)
+    # timed steps (default launch order: size buckets in series on one stream)
     e0, e1 = _device.Event(), _device.Event()
     step_ms = []
     _device.synchronize()

@@ -97,6 +97,97 @@ __global__ __launch_bounds__(NT) void topk_kernel(const CurveGeom* __restrict__ 
     }
 }
 
+// Top-k for k <= KM in one read of the powers: each thread keeps a sorted
+// list of its KM best (value desc, index asc) over a strided slice, then k
+// rounds of block argmax over the lists' heads pick the winners in order.
+// Same order and ties as topk_kernel.
+template <int KM>
+__global__ __launch_bounds__(NT) void topk_small_kernel(const CurveGeom* __restrict__ geom,
+                                                        const float* __restrict__ power,
+                                                        const uint8_t* __restrict__ mask,
+                                                        int K, float* __restrict__ top_power,
+                                                        int32_t* __restrict__ top_index) {
+    __shared__ float sv[NT / kWave];
+    __shared__ int si[NT / kWave];
+    __shared__ int win;
+    const int c = blockIdx.x;
+    const CurveGeom& G = geom[c];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    auto better = [](float v, int i, float bv, int bi) { return v > bv || (v == bv && i < bi); };
+    float lv[KM];
+    int li[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+        lv[q] = -INFINITY;
+        li[q] = 0x7fffffff;
+    }
+    for (int k = tid; k < G.nf; k += NT) {
+        if (mask && mask[G.out + k]) continue;
+        const float v = power[G.out + k];
+        if (!(v >= 0.f)) continue;   // -1 sentinel and NaN
+        if (!better(v, k, lv[KM - 1], li[KM - 1])) continue;
+        // insert into the sorted list (registers: unrolled shift)
+        float cv = v;
+        int ci = k;
+#pragma unroll
+        for (int q = 0; q < KM; ++q) {
+            if (better(cv, ci, lv[q], li[q])) {
+                const float tv = lv[q];
+                const int ti = li[q];
+                lv[q] = cv;
+                li[q] = ci;
+                cv = tv;
+                ci = ti;
+            }
+        }
+    }
+    int head = 0;   // next unpicked entry of this thread's list
+    for (int round = 0; round < K; ++round) {
+        float best = -INFINITY;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < KM; ++q)
+            if (q == head) {
+                best = lv[q];
+                bi = li[q];
+            }
+        const int mine = bi;   // this thread's head (indices are unique per curve)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(best, o, kWave);
+            const int oi = __shfl_xor(bi, o, kWave);
+            if (better(ov, oi, best, bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            sv[wv] = best;
+            si[wv] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float bv = sv[0];
+            int bx = si[0];
+            for (int q = 1; q < NT / kWave; ++q)
+                if (better(sv[q], si[q], bv, bx)) {
+                    bv = sv[q];
+                    bx = si[q];
+                }
+            const bool ok = bx != 0x7fffffff;
+            top_power[(int64_t)c * K + round] = ok ? bv : -1.f;
+            top_index[(int64_t)c * K + round] = ok ? bx : -1;
+            win = bx;
+        }
+        __syncthreads();
+        // the thread whose head won moves on to its next entry
+        if (win != 0x7fffffff && mine == win) ++head;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 hipError_t launch_gaussian_grid(const double* t, const double* y, int n_obs, int ng, int m,
@@ -115,8 +206,12 @@ hipError_t launch_topk(const CurveGeom* d_geom, int ncurves, const float* power,
                        const uint8_t* mask, int k, float* top_power, int32_t* top_index,
                        hipStream_t s) {
     if (ncurves <= 0 || k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(topk_kernel, dim3(ncurves), dim3(NT), 0, s, d_geom, power, mask, k,
-                       top_power, top_index);
+    if (k <= 8)
+        hipLaunchKernelGGL((topk_small_kernel<8>), dim3(ncurves), dim3(NT), 0, s, d_geom, power,
+                           mask, k, top_power, top_index);
+    else
+        hipLaunchKernelGGL(topk_kernel, dim3(ncurves), dim3(NT), 0, s, d_geom, power, mask, k,
+                           top_power, top_index);
     return hipGetLastError();
 }
 

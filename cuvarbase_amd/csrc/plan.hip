@@ -35,6 +35,7 @@ struct gls_plan {
     };
     std::vector<Bucket> buckets;   // one entry per chunk of a size bucket
     std::vector<int> bucket_lane;  // 0: the caller's stream, j > 0: auxiliary stream j-1
+    bool fork_buckets = false;     // run the lanes concurrently (gls_plan_set_fork)
     std::vector<hipEvent_t> fork_join;  // fork event + one join event per aux stream
     std::vector<int32_t> bcurves;  // curve indices of the entries, in order
     int64_t chunk_bytes = 0;       // > 0: grid region reused by every entry
@@ -416,7 +417,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         // when timing, whose events need one serial stream), join at the end
         bool fork = false;
         for (int lane : P->bucket_lane) fork |= lane > 0;
-        fork &= !P->timing;
+        fork &= P->fork_buckets && !P->timing;
         hipStream_t lanes[kAuxStreams + 1] = {s};
         if (fork) {
             GLS_HIP_TRY(aux_streams(lanes + 1));
@@ -750,6 +751,12 @@ int gls_gaussian_grid(const double* t, const double* y, int32_t n_obs, int32_t n
     if (!(tmax > tmin) || !(spp > 0)) return fail(GLS_E_INVALID, "need tmax > tmin and spp > 0");
     GLS_HIP_TRY(launch_gaussian_grid(t, y, n_obs, ng, m, b, tmin, tmax, spp, grid,
                                      (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_plan_set_fork(gls_plan* plan, int32_t enable) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    plan->fork_buckets = enable != 0;
     return GLS_OK;
 }
 

@@ -87,6 +87,11 @@ class Plan:
                   C.byref(n2), C.byref(b))
         return n.value, n1.value, n2.value, b.value
 
+    def set_fork(self, enable=True):
+        """Run independent size buckets concurrently on the library's streams."""
+        _lib.call("gls_plan_set_fork", self.handle, C.c_int32(1 if enable else 0))
+        return self
+
     def enable_timing(self, enable=True):
         _lib.call("gls_plan_enable_timing", self.handle, C.c_int32(1 if enable else 0))
 
