@@ -1,0 +1,212 @@
+"""Host logic of the continuous-batching server against a simulated device.
+
+The libgls entry points the server calls are replaced by a host stand-in in
+which every stream is a queue that runs only when an event on it is queried
+(at random) or synchronised, so work completes late and out of order as on a
+GPU: a server that reused a staging buffer, a result block or an arena block
+too early, mixed up offsets between requests, or retired a batch before its
+copies ran would produce wrong periodograms here.  The simulated execute
+computes each curve with the FP64 oracle's direct sums; top-k uses a stable
+argsort.  No GPU is involved.
+"""
+import ctypes as C
+import io
+import random
+
+import numpy as np
+import pytest
+
+from oracle import gls as og
+
+
+class FakeDevice:
+    def __init__(self, seed=1):
+        self.rng = random.Random(seed)
+        self.keep, self.plans, self.streams, self.events = {}, {}, {}, {}
+        self.next_plan = 1000
+
+    def _enq(self, s, fn):
+        self.streams.setdefault(s, []).append(fn)
+
+    def _run(self, s):
+        q = self.streams.get(s, [])
+        while q:
+            q.pop(0)()
+
+    def _arr(self, ctype, addr, n):
+        return np.ctypeslib.as_array((ctype * n).from_address(addr)) if n else np.zeros(0)
+
+    def call(self, name, *args):
+        a = [x.value if isinstance(x, (C.c_int32, C.c_size_t, C.c_void_p, C.c_double)) else x
+             for x in args]
+        if name == "gls_init":
+            return 0
+        if name == "gls_stream_create":
+            h = 0x10000 + len(self.streams) + 1
+            self.streams[h] = []
+            args[0]._obj.value = h
+            return 0
+        if name == "gls_event_create":
+            h = 0x20000 + len(self.events) + 1
+            self.events[h] = [True, None]
+            args[0]._obj.value = h
+            return 0
+        if name in ("gls_stream_destroy", "gls_event_destroy", "gls_free", "gls_host_free"):
+            return 0
+        if name == "gls_event_record":
+            ev, s = a
+            self.events[ev] = [False, s]
+            self._enq(s, lambda: self.events[ev].__setitem__(0, True))
+            return 0
+        if name == "gls_event_query":
+            ev = a[0]
+            if not self.events[ev][0] and self.rng.random() < 0.3:
+                self._run(self.events[ev][1])
+            args[1]._obj.value = int(self.events[ev][0])
+            return 0
+        if name == "gls_stream_synchronize":
+            self._run(a[0])
+            return 0
+        if name in ("gls_device_synchronize", "gls_event_synchronize"):
+            for s in list(self.streams):
+                self._run(s)
+            return 0
+        if name in ("gls_malloc", "gls_host_alloc"):
+            buf = C.create_string_buffer(max(int(a[1]), 1))
+            self.keep[C.addressof(buf)] = buf
+            args[0]._obj.value = C.addressof(buf)
+            return 0
+        if name.startswith("gls_memcpy"):
+            dst, src, n, s = a
+            self._enq(s, lambda: C.memmove(dst, src, n))
+            return 0
+        if name == "gls_lomb_plan_create":
+            h, arr, n, prm = args
+            self.next_plan += 1
+            self.plans[self.next_plan] = dict(
+                curves=[(c.obs_offset, c.n_obs, c.nf, c.f0, c.df, c.out_offset)
+                        for c in (arr[i] for i in range(n.value))],
+                mode=prm._obj.mode, alive=True)
+            h._obj.value = self.next_plan
+            return 0
+        if name == "gls_plan_workspace_bytes":
+            args[1]._obj.value = 4096
+            return 0
+        if name == "gls_plan_destroy":
+            self.plans[a[0]]["alive"] = False
+            return 0
+        if name in ("gls_lomb_execute", "gls_lomb_execute_custom"):
+            if name == "gls_lomb_execute":
+                h, t, y, dy, pw, ws, s = a
+                fr = None
+            else:
+                h, t, y, dy, fr, pw, _f64, ws, s = a
+
+            def run():
+                P = self.plans[h]
+                assert P["alive"], "plan destroyed before its execute ran"
+                C.memset(ws, 0x7F, 4096)   # a shared workspace would be clobbered
+                for (o, n, nf, f0, df, oo) in P["curves"]:
+                    tt = self._arr(C.c_double, t + 8 * o, n).copy()
+                    yy = self._arr(C.c_double, y + 8 * o, n).copy()
+                    dd = self._arr(C.c_double, dy + 8 * o, n).copy()
+                    f = (f0 + df * np.arange(nf) if fr is None
+                         else self._arr(C.c_double, fr + 8 * oo, nf).copy())
+                    self._arr(C.c_float, pw + 4 * oo, nf)[:] = og.gls_direct(
+                        tt, yy, dd, f, mode=P["mode"])
+            self._enq(s, run)
+            return 0
+        if name == "gls_topk":
+            h, ws, pw, mask, k, tp, ti, s = a
+
+            def run():
+                for ci, (o, n, nf, f0, df, oo) in enumerate(self.plans[h]["curves"]):
+                    p = self._arr(C.c_float, pw + 4 * oo, nf).copy()
+                    order = np.argsort(-p, kind="stable")[:k]
+                    self._arr(C.c_float, tp + 4 * ci * k, k)[:] = p[order]
+                    self._arr(C.c_int32, ti + 4 * ci * k, k)[:] = order
+            self._enq(s, run)
+            return 0
+        raise AssertionError(f"unexpected libgls call {name}")
+
+
+@pytest.fixture
+def serve(monkeypatch):
+    from cuvarbase_amd import _lib, serve as srvmod
+
+    dev = FakeDevice()
+    monkeypatch.setattr(_lib, "call", dev.call)
+    monkeypatch.setattr(_lib, "init", lambda device=0: None)
+    monkeypatch.setattr(srvmod.time, "sleep", lambda s: None)
+    return srvmod
+
+
+def _requests(n, seed=3):
+    rng = np.random.default_rng(seed)
+    data, freqs = [], []
+    for i in range(n):
+        m = int(rng.integers(20, 80))
+        t = np.sort(rng.uniform(0, 20, m))
+        data.append((t, np.cos(t * (2 + i)) + rng.normal(0, 0.3, m), np.full(m, 0.3)))
+        freqs.append(0.05 + 0.01 * np.arange(int(rng.integers(200, 400))))
+    return data, freqs
+
+
+@pytest.mark.parametrize("zero_copy", [False, True])
+def test_stream_results_match_oracle(serve, zero_copy):
+    data, freqs = _requests(24)
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=3, max_batch_evals=700,
+                          zero_copy=zero_copy, result_buffer_bytes=1 << 16)
+    st = srv.serve_stream(np.zeros(len(data)), data, freqs)
+    assert st["batches"] > 3
+    for r, d, f in zip(st["requests_obj"], data, freqs):
+        np.testing.assert_allclose(r.power, og.gls_direct(*d, f), atol=1e-6)
+        r.release()
+    if zero_copy:
+        assert srv.host_arena.used == 0
+
+
+def test_records_mixed_classes_and_release(serve):
+    from cuvarbase_amd import wire
+
+    data, freqs = _requests(18, seed=4)
+    rng = np.random.default_rng(9)
+    recs = []
+    for i, (d, f) in enumerate(zip(data, freqs)):
+        if i % 3 == 2:
+            recs.append(wire.RequestRecord(i, *d, freqs=np.sort(rng.uniform(0.05, 3, 150))))
+        else:
+            recs.append(wire.RequestRecord(i, *d, f0=f[0], df=f[1] - f[0], nf=len(f),
+                                           k=3 if i % 3 == 1 else 0))
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=2, max_batch_evals=700, zero_copy=True,
+                          result_buffer_bytes=1 << 14)
+    dst = io.BytesIO()
+    src = io.BytesIO(b"".join(wire.encode_request(r) for r in recs))
+    assert srv.serve_records(src, dst) == len(recs)
+    assert srv.host_arena.used == 0
+    dst.seek(0)
+    out = {r.id: r for r in wire.iter_results(dst)}
+    for rq in recs:
+        ref = og.gls_direct(rq.t, rq.y, rq.dy, rq.frequencies())
+        o = out[rq.id]
+        if rq.k:
+            order = np.argsort(-ref.astype(np.float32), kind="stable")[:3]
+            np.testing.assert_array_equal(o.top_index, order)
+            np.testing.assert_allclose(o.top_freq, rq.frequencies()[order])
+            assert np.all((o.fap >= 0) & (o.fap <= 1))
+        else:
+            np.testing.assert_allclose(o.power, ref, atol=1e-6)
+
+
+def test_unreleased_results_block_admission(serve):
+    data, freqs = _requests(6, seed=5)
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=1, max_batch_evals=300, zero_copy=True,
+                          result_buffer_bytes=4 * 420)
+    st = srv.serve_stream(np.zeros(1), data[:1], freqs[:1])
+    assert srv.host_arena.used > 0          # the first result is still held
+    t, y, dy = data[1]
+    srv.submit(t, y, dy, freqs[1])
+    with pytest.raises(MemoryError, match="release"):
+        srv.drain()
+    st["requests_obj"][0].release()
+    assert srv.host_arena.used == 0
