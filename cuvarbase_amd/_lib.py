@@ -117,6 +117,9 @@ _PROTOS = {
     "gls_lomb_execute_custom": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp],
     "gls_mh_power": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32, _vp],
     "gls_plan_set_fork": [_vp, C.c_int32],
+    "gls_plan_graph_create": [_vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
+    "gls_graph_launch": [_vp, _vp],
+    "gls_graph_destroy": [_vp],
     "gls_nfft_plan_create": [C.POINTER(_vp), C.POINTER(gls_nfft_item), C.c_int32,
                              C.c_double, C.c_int32],
     "gls_nfft_execute": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],

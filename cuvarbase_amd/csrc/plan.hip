@@ -389,9 +389,10 @@ PointArrays point_arrays(const gls_plan* P, void* ws) {
 
 // The whole launch sequence of a plan on stream s.
 int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* out,
-             int out_f64, int y_is_f32, void* ws, hipStream_t s, const double* freqs = nullptr) {
+             int out_f64, int y_is_f32, void* ws, hipStream_t s, const double* freqs = nullptr,
+             bool upload_meta = true) {
     char* base = (char*)ws;
-    if (!P->meta.empty())
+    if (upload_meta && !P->meta.empty())
         GLS_HIP_TRY(hipMemcpyAsync(base, P->meta.data(), P->meta.size(), hipMemcpyHostToDevice,
                                    s));
     const CurveGeom* d_geom = (const CurveGeom*)(base + P->off_geom);
@@ -804,6 +805,73 @@ int gls_topk(const gls_plan* plan, const void* workspace, const float* power,
     const CurveGeom* d_geom = (const CurveGeom*)((const char*)workspace + plan->off_geom);
     GLS_HIP_TRY(launch_topk(d_geom, plan->ncurves, power, mask, k, top_power, top_index,
                             (hipStream_t)stream));
+    return GLS_OK;
+}
+
+// ---- captured executes ------------------------------------------------------
+struct gls_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a, const void* b,
+                          void* power, int32_t power_is_f64, void* workspace, void* stream,
+                          gls_graph** out) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan || !out) return fail(GLS_E_INVALID, "plan or out is NULL");
+    *out = nullptr;
+    if (plan->kind != 0 || plan->pp.method == GLS_METHOD_DIRECT_CUSTOM)
+        return fail(GLS_E_UNSUPPORTED, "graphs are made of Lomb-Scargle NFFT or direct plans");
+    if (plan->ncurves == 0) return fail(GLS_E_INVALID, "empty plan");
+    if (!t || !a || !b || !power || !workspace) return fail(GLS_E_INVALID, "NULL device pointer");
+    const hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(GLS_E_INVALID, "capture needs a created stream (not the null stream)");
+    // descriptors go into the workspace once, outside the graph
+    if (!plan->meta.empty())
+        GLS_HIP_TRY(hipMemcpyAsync(workspace, plan->meta.data(), plan->meta.size(),
+                                   hipMemcpyHostToDevice, s));
+    GLS_HIP_TRY(hipStreamSynchronize(s));
+    const bool timing = plan->timing, fork = plan->fork_buckets;
+    plan->timing = false;
+    plan->fork_buckets = false;
+    gls_graph* g = new gls_graph();
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = run_plan(plan, t, a, b, power, power_is_f64 ? 1 : 0, 0, workspace, s, nullptr,
+                      false);
+        hipError_t e2 = hipStreamEndCapture(s, &g->graph);
+        if (rc == GLS_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+    } else {
+        rc = hip_fail(e, "hipStreamBeginCapture");
+    }
+    plan->timing = timing;
+    plan->fork_buckets = fork;
+    if (rc == GLS_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+    if (rc != GLS_OK) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return GLS_OK;
+}
+
+int gls_graph_launch(gls_graph* graph, void* stream) {
+    if (!graph || !graph->exec) return fail(GLS_E_INVALID, "graph is NULL");
+    GLS_HIP_TRY(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_graph_destroy(gls_graph* graph) {
+    if (!graph) return GLS_OK;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
     return GLS_OK;
 }
 

@@ -122,6 +122,27 @@ class Plan:
             pass
 
 
+class Graph:
+    """Owns a ``gls_graph`` (a captured execute)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream):
+        _lib.call("gls_graph_launch", self.handle, _handle(stream))
+
+    def destroy(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h and _lib._lib is not None:
+            _lib._lib.gls_graph_destroy(C.c_void_p(h))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:  # pragma: no cover
+            pass
+
+
 class LombBatch:
     """Many light curves, planned and run together.
 
@@ -253,6 +274,20 @@ class LombBatch:
         flat = self.download_async(stream=s)
         s.synchronize()
         return [np.array(p) for p in self.split(flat)]
+
+    def capture(self, stream):
+        """A HIP graph of this batch's execute on its current device buffers
+        (``allocate``/``upload`` first); ``launch(stream)`` reruns it on the
+        current contents of the inputs."""
+        if self.custom:
+            raise NotImplementedError("custom-frequency batches are not captured")
+        if self.d_t is None:
+            self.allocate()
+        g = C.c_void_p()
+        _lib.call("gls_plan_graph_create", self.plan.handle, self.d_t.ptr, self.d_a.ptr,
+                  self.d_b.ptr, self.d_power.ptr, C.c_int32(1 if self.use_double else 0),
+                  self.d_ws.ptr, _handle(stream), C.byref(g))
+        return Graph(g.value)
 
     def topk(self, k, mask=None, stream=None):
         """Top-k (power, index) per curve from the device result."""

@@ -202,6 +202,22 @@ int gls_lomb_execute(gls_plan* plan, const double* t, const void* a,
  * overlap.  Off by default.  No reference counterpart. */
 int gls_plan_set_fork(gls_plan* plan, int32_t enable);
 
+/* Captured executes: the kernel sequence of gls_lomb_execute(_f64) on fixed
+ * pointers as a HIP graph, for launch-bound loops (the reference launches
+ * nine kernels per curve from Python, cuvarbase/lombscargle.py:579-689).
+ * Creation copies the plan's descriptors into `workspace` (once) and
+ * captures the launches on `stream` (a created stream); every
+ * gls_graph_launch then recomputes the powers from the current contents of
+ * t, a, b.  The workspace belongs to the graph until it is destroyed.  The
+ * plan may be destroyed after creation.  Lomb-Scargle plans of methods NFFT
+ * and DIRECT. */
+typedef struct gls_graph gls_graph;   /* opaque */
+int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a,
+                          const void* b, void* power, int32_t power_is_f64,
+                          void* workspace, void* stream, gls_graph** graph);
+int gls_graph_launch(gls_graph* graph, void* stream);
+int gls_graph_destroy(gls_graph* graph);
+
 /* Custom-frequency plans (GLS_METHOD_DIRECT_CUSTOM): curve i is evaluated
  * at freqs[out_offset .. out_offset + nf) (device f64; f0/df ignored) into
  * f32 (power_is_f64 = 0) or f64 powers.  Replaces lomb_dirsum_custom_frq

@@ -455,3 +455,37 @@ def test_forked_buckets_match_serial():
     worst = max(float(np.max(np.abs(x.astype(float) - y))) for x, y in zip(pa, pb))
     worst2 = max(float(np.max(np.abs(x.astype(float) - y))) for x, y in zip(pb, pb2))
     assert worst < 1e-6 and worst2 < 1e-6
+
+
+def test_captured_graph_replays_match_eager():
+    """A captured execute replayed several times, with other eager work and
+    synchronisation between replays, equals the eager execute each time, and
+    follows new input values written into the same buffers."""
+    from cuvarbase_amd import engine, synthetic
+    from cuvarbase_amd._device import Stream
+
+    data, freqs = synthetic.survey(12, seed=26)
+    b = engine.LombBatch(data, freqs, sigma=2, m=8)
+    s = Stream()
+    b.upload(stream=s)
+    s.synchronize()
+    g = b.capture(s)
+    eager = engine.LombBatch(data, freqs, sigma=2, m=8).run()
+    for rep in range(3):
+        g.launch(s)
+        s.synchronize()
+        got = b.split(b.d_power.get(count=b.total_out))
+        worst = max(float(np.max(np.abs(x - y))) for x, y in zip(got, eager))
+        assert worst < 1e-6, (rep, worst)
+        engine.LombBatch(data[:2], freqs[:2], sigma=2, m=8).run()   # eager work between
+    # new values in the same buffers (same shapes)
+    rng = np.random.default_rng(3)
+    data2 = [(t, y[::-1].copy() + rng.normal(0, 0.1, len(y)), dy) for (t, y, dy) in data]
+    b2 = engine.LombBatch(data2, freqs, sigma=2, m=8)
+    b.d_a.set_async(b2._host[1], stream=s)
+    g.launch(s)
+    s.synchronize()
+    got = b.split(b.d_power.get(count=b.total_out))
+    ref = b2.run()
+    assert max(float(np.max(np.abs(x - y))) for x, y in zip(got, ref)) < 1e-6
+    g.destroy()
