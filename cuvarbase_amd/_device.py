@@ -40,7 +40,7 @@ class Stream:
         h, self.handle = getattr(self, "handle", None), None
         if h is not None and _lib._lib is not None:
             try:
-                _lib._lib.gls_stream_destroy(h)
+                _lib.call("gls_stream_destroy", h)
             except Exception:  # pragma: no cover - interpreter shutdown
                 pass
 
@@ -73,7 +73,7 @@ class Event:
         h, self.handle = getattr(self, "handle", None), None
         if h is not None and _lib._lib is not None:
             try:
-                _lib._lib.gls_event_destroy(h)
+                _lib.call("gls_event_destroy", h)
             except Exception:  # pragma: no cover
                 pass
 
@@ -163,7 +163,7 @@ class DeviceArray:
     def free(self):
         p, self.ptr = getattr(self, "ptr", 0), 0
         if p and _lib._lib is not None:
-            _lib._lib.gls_free(C.c_void_p(p))
+            _lib.call("gls_free", C.c_void_p(p))
 
     def __del__(self):
         try:
@@ -182,7 +182,7 @@ class _HostBlock:
         p, self.ptr = getattr(self, "ptr", None), None
         if p and _lib._lib is not None:
             try:
-                _lib._lib.gls_host_free(C.c_void_p(p))
+                _lib.call("gls_host_free", C.c_void_p(p))
             except Exception:  # pragma: no cover
                 pass
 

@@ -155,57 +155,53 @@ __global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __re
 // 1e-4.  These few frequencies are recomputed by direct sums with FP64
 // phases and FP64 accumulation (one block per curve, one frequency per
 // thread, points staged through LDS).
-// Thread layout: 16 frequency slots x 16 point lanes; a slot's 16 lanes sum
-// over the points with stride 16 and are reduced by shuffles.
+// Grid (curve, group of 4 frequencies); each wave of a block takes one
+// frequency and its 64 lanes stride over the curve's points (FP64 phases,
+// FP64 sums), then reduce by shuffles.  Groups past the curve's low
+// frequencies exit at once.
+constexpr int kLowPerBlock = NT / kWave;
+
 template <int OUT>
 __global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict__ geom,
                                                      const CurveState* __restrict__ state,
                                                      PointArrays pa, PlanParams pp,
                                                      void* __restrict__ out) {
-    constexpr int LANES = 16;
-    constexpr int SLOTS = NT / LANES;
     const int tid = threadIdx.x;
-    const int lane = tid % LANES;
-    const int slot = tid / LANES;
+    const int lane = tid & (kWave - 1);
+    const int k = blockIdx.y * kLowPerBlock + tid / kWave;
     const CurveGeom& G = geom[blockIdx.x];
-    const CurveState st = state[blockIdx.x];
+    const CurveState& st = state[blockIdx.x];
+    // frequency k is low if it and every frequency before it span fewer than
+    // kLowCycles cycles over the baseline (the leading run of the grid)
+    const int first = blockIdx.y * kLowPerBlock;
+    if (first >= min(G.nf, kLowMax)) return;
+    if (fabs(G.f0 + (double)first * G.df) * st.tspan >= kLowCycles) return;
+    bool active = k < min(G.nf, kLowMax);
+    for (int q = first; q <= k && active; ++q)
+        active = fabs(G.f0 + (double)q * G.df) * st.tspan < kLowCycles;
+    if (!active) return;   // whole waves only: k is uniform within a wave
+    const double f = G.f0 + (double)k * G.df;
     const int N = G.n_obs;
-    // number of leading frequencies with |f| (t_max - t_min) < kLowCycles
-    int nlow = 0;
-    {
-        const int lim = min(G.nf, kLowMax);
-        while (nlow < lim && fabs(G.f0 + (double)nlow * G.df) * st.tspan < kLowCycles) ++nlow;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = lane; j < N; j += kWave) {
+        const double x = f * pa.dt[G.pts + j];
+        double sn, cs;
+        sincospi(2.0 * (x - rint(x)), &sn, &cs);
+        const double w = pa.w[G.pts + j], yw = pa.yw[G.pts + j];
+        acc[0] += w * cs;
+        acc[1] += w * sn;
+        acc[2] += w * (cs * cs - sn * sn);
+        acc[3] += w * (2.0 * cs * sn);
+        acc[4] += yw * cs;
+        acc[5] += yw * sn;
     }
-    for (int k0 = 0; k0 < nlow; k0 += SLOTS) {
-        const int k = k0 + slot;
-        const bool active = k < nlow;
-        const double f = G.f0 + (double)k * G.df;
-        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (active) {
-            for (int j = lane; j < N; j += LANES) {
-                const double x = f * pa.dt[G.pts + j];
-                double sn, cs;
-                sincospi(2.0 * (x - rint(x)), &sn, &cs);
-                const double w = pa.w[G.pts + j], yw = pa.yw[G.pts + j];
-                acc[0] += w * cs;
-                acc[1] += w * sn;
-                acc[2] += w * (cs * cs - sn * sn);
-                acc[3] += w * (2.0 * cs * sn);
-                acc[4] += yw * cs;
-                acc[5] += yw * sn;
-            }
-        }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-#pragma unroll
-            for (int o = LANES / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, LANES);
-        }
-        if (active && lane == 0) {
-            const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
-                                       st.ybar, pp.mode, pp.reg_c, pp.reg_s);
-            if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
-            else ((double*)out)[G.out + k] = P;
-        }
+    for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+        const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
+                                   st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+        if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+        else ((double*)out)[G.out + k] = P;
     }
 }
 
@@ -229,12 +225,13 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const CurveState* state, int 
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
                           hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
+    const dim3 grid(ncurves, (kLowMax + kLowPerBlock - 1) / kLowPerBlock);
     if (out_f64)
-        hipLaunchKernelGGL(lowfreq_kernel<1>, dim3(ncurves), dim3(NT), 0, s, d_geom, state, pa,
-                           pp, power);
+        hipLaunchKernelGGL(lowfreq_kernel<1>, grid, dim3(NT), 0, s, d_geom, state, pa, pp,
+                           power);
     else
-        hipLaunchKernelGGL(lowfreq_kernel<0>, dim3(ncurves), dim3(NT), 0, s, d_geom, state, pa,
-                           pp, power);
+        hipLaunchKernelGGL(lowfreq_kernel<0>, grid, dim3(NT), 0, s, d_geom, state, pa, pp,
+                           power);
     return hipGetLastError();
 }
 

@@ -112,7 +112,10 @@ constexpr int pass2_rows(int p2) {
 }
 
 // launchers (defined in the kernel translation units)
-hipError_t launch_prep(const CurveGeom* d_geom, int ncurves, CurveState* state,
+// curves of more than kWidePrep points are prepared by 1024-thread blocks
+constexpr int kWidePrep = 16384;
+hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall,
+                       const int32_t* wide, int nwide, CurveState* state,
                        const double* t, const void* a, const void* b,
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
                        int y_is_f32, hipStream_t s);

@@ -43,7 +43,9 @@ struct gls_plan {
     int rf = 1;
     // host image of the descriptors uploaded into workspace[0 .. meta.size())
     std::vector<char> meta;
-    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0, off_estab = 0;
+    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0, off_estab = 0, off_prep = 0;
+    std::vector<int32_t> prep_list;   // curves prepared by 256-thread blocks, then wide ones
+    int nprep_small = 0;
     std::vector<float> estab;      // ES deconvolution table (empty for the Gaussian)
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
@@ -341,7 +343,15 @@ void build_stage_list(gls_plan* P) {
 
 // lays out [geom | bucket curves | tiles] at the start of the workspace
 void reserve_meta(gls_plan* P, Layout& L, size_t nbcurves, size_t ntiles_ints) {
+    // prep order: curves up to kWidePrep points first, then the long ones
+    P->prep_list.clear();
+    for (int i = 0; i < P->ncurves; ++i)
+        if (P->geom[i].n_obs <= kWidePrep) P->prep_list.push_back(i);
+    P->nprep_small = (int)P->prep_list.size();
+    for (int i = 0; i < P->ncurves; ++i)
+        if (P->geom[i].n_obs > kWidePrep) P->prep_list.push_back(i);
     P->off_geom = L.take(sizeof(CurveGeom) * P->geom.size());
+    P->off_prep = L.take(sizeof(int32_t) * P->prep_list.size());
     P->off_bcurves = L.take(sizeof(int32_t) * nbcurves);
     P->off_estab = L.take(sizeof(float) * P->estab.size());
     P->off_tiles = L.take(sizeof(int32_t) * ntiles_ints);
@@ -353,6 +363,9 @@ void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
     if (!P->estab.empty())
         std::memcpy(P->meta.data() + P->off_estab, P->estab.data(),
                     sizeof(float) * P->estab.size());
+    if (!P->prep_list.empty())
+        std::memcpy(P->meta.data() + P->off_prep, P->prep_list.data(),
+                    sizeof(int32_t) * P->prep_list.size());
     if (!P->geom.empty())
         std::memcpy(P->meta.data() + P->off_geom, P->geom.data(),
                     sizeof(CurveGeom) * P->geom.size());
@@ -411,7 +424,10 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
     PlanParams pp = P->pp;
     pp.es_deconv = P->estab.empty() ? nullptr : (const float*)(base + P->off_estab);
     GLS_HIP_TRY(mark());
-    GLS_HIP_TRY(launch_prep(d_geom, P->ncurves, state, t, a, b, pa, bkt, P->pp, y_is_f32, s));
+    const int32_t* d_prep = (const int32_t*)(base + P->off_prep);
+    GLS_HIP_TRY(launch_prep(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
+                            P->ncurves - P->nprep_small, state, t, a, b, pa, bkt, pp, y_is_f32,
+                            s));
     GLS_HIP_TRY(mark());
     if (P->pp.method == GLS_METHOD_NFFT) {
         // fork the buckets over the caller's and the auxiliary streams (not

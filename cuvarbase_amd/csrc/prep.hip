@@ -1,6 +1,7 @@
 // Per-curve data preparation and point bucketing.
 //
-// One workgroup per curve:
+// One workgroup per curve (1024 threads for curves of more than 16,384
+// points, 256 otherwise):
 //   phase A  reductions: tmin/tmax, and for raw input the weighted mean and
 //            variance of LombScargleMemory.setdata (cuvarbase/lombscargle.py:
 //            243-257) after the nanmean centring of normalize_light_curves
@@ -44,24 +45,67 @@ __device__ __forceinline__ PrepVals load_point(const double* t, const void* a, c
     return p;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ geom,
-                                                  CurveState* __restrict__ state,
-                                                  const double* __restrict__ t,
-                                                  const void* __restrict__ a,
-                                                  const void* __restrict__ b,
-                                                  PointArrays pa, int32_t* __restrict__ bkt,
-                                                  int m, int method, int y_is_f32) {
-    __shared__ double red[NT / kWave];
-    __shared__ int hist[4096];
-    const int c = blockIdx.x;
+// In-place exclusive scan of h[0 .. nb) by the whole block; returns the total.
+template <int NTP>
+__device__ int block_exclusive_scan(int* h, int nb, int* wsum) {
+    const int tid = threadIdx.x;
+    const int per = (nb + NTP - 1) / NTP;
+    const int lo = min(nb, tid * per), hi = min(nb, lo + per);
+    int local = 0;
+    for (int i = lo; i < hi; ++i) local += h[i];
+    // inclusive scan of the chunk sums over the block
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    int v = local;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int u = __shfl_up(v, o, kWave);
+        if (lane >= o) v += u;
+    }
+    if (lane == kWave - 1) wsum[wv] = v;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int w = 0; w < NTP / kWave; ++w) {
+            const int x = wsum[w];
+            wsum[w] = acc;
+            acc += x;
+        }
+        wsum[NTP / kWave] = acc;
+    }
+    __syncthreads();
+    int run = v - local + wsum[wv];   // exclusive prefix of this chunk
+    for (int i = lo; i < hi; ++i) {
+        const int x = h[i];
+        h[i] = run;
+        run += x;
+    }
+    const int total = wsum[NTP / kWave];
+    __syncthreads();
+    return total;
+}
+
+// NTP threads per curve; curves taken from list[blockIdx.x] (256 threads for
+// most curves, 1024 for long ones)
+template <int KIND, int NTP>
+__global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__ geom,
+                                                   const int32_t* __restrict__ list,
+                                                   CurveState* __restrict__ state,
+                                                   const double* __restrict__ t,
+                                                   const void* __restrict__ a,
+                                                   const void* __restrict__ b,
+                                                   PointArrays pa, int32_t* __restrict__ bkt,
+                                                   int m, int method, int y_is_f32) {
+    __shared__ double red[NTP / kWave];
+    __shared__ int hist[2][4096];
+    __shared__ int wsum[NTP / kWave + 1];
+    const int c = list[blockIdx.x];
     const CurveGeom G = geom[c];
     const int tid = threadIdx.x;
     const int N = G.n_obs;
 
     // ---- phase A: reductions ------------------------------------------------
     double tmin = INFINITY, tmax = -INFINITY, sy = 0.0, cnt = 0.0, s0 = 0.0;
-    for (int j = tid; j < N; j += NT) {
+    for (int j = tid; j < N; j += NTP) {
         PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
         tmin = fmin(tmin, p.t);
         tmax = fmax(tmax, p.t);
@@ -73,28 +117,28 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
             s0 += 1.0 / (p.wv * p.wv);
         }
     }
-    tmin = block_min<NT>(tmin, red);
-    tmax = block_max<NT>(tmax, red);
+    tmin = block_min<NTP>(tmin, red);
+    tmax = block_max<NTP>(tmax, red);
     double ymean = 0.0, ybar = G.ybar_in, yy = G.yy_in, s0inv = 1.0;
     if (KIND == 0) {
-        sy = block_sum<NT>(sy, red);
-        cnt = block_sum<NT>(cnt, red);
-        s0 = block_sum<NT>(s0, red);
+        sy = block_sum<NTP>(sy, red);
+        cnt = block_sum<NTP>(cnt, red);
+        s0 = block_sum<NTP>(s0, red);
         ymean = sy / cnt;
         s0inv = 1.0 / s0;
         double s1 = 0.0;
-        for (int j = tid; j < N; j += NT) {
+        for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             s1 += (p.yv - ymean) / (p.wv * p.wv);
         }
-        ybar = block_sum<NT>(s1, red) * s0inv;
+        ybar = block_sum<NTP>(s1, red) * s0inv;
         double s2 = 0.0;
-        for (int j = tid; j < N; j += NT) {
+        for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             const double d = p.yv - ymean - ybar;
             s2 += d * d / (p.wv * p.wv);
         }
-        yy = block_sum<NT>(s2, red) * s0inv;
+        yy = block_sum<NTP>(s2, red) * s0inv;
     }
     if (tid == 0) {
         CurveState st;
@@ -121,7 +165,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
     };
 
     if (method == GLS_METHOD_DIRECT || method == GLS_METHOD_DIRECT_CUSTOM) {
-        for (int j = tid; j < N; j += NT) {
+        for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             double w, yw;
             weights_of(p, w, yw);
@@ -132,66 +176,65 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
         return;
     }
 
-    // prepared points for the low-frequency direct sums of lomb NFFT plans
-    if (KIND != 2 && pa.dt != nullptr) {
-        for (int j = tid; j < N; j += NT) {
-            PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
-            double w, yw;
-            weights_of(p, w, yw);
-            const double dt = p.t - tref;
-            pa.w[G.pts + j] = isfinite(dt) ? (float)w : 0.f;
-            pa.yw[G.pts + j] = isfinite(dt) ? (float)yw : 0.f;
-            pa.dt[G.pts + j] = isfinite(dt) ? dt : 0.0;
-        }
-    }
-
-    // ---- phase B: cells, modulation and bucketing -------------------------
+    // ---- phase B: cells, modulation and bucketing of both node sets --------
+    // (x = df (t - tref) for SYW and SW(f), 2x for SW(2f)); one counting pass,
+    // one block scan per set, one scatter pass
     const int n = G.n;
     const int p2 = G.p2;
     const int n2 = 1 << p2;
     const int c1 = G.c1;
     const int nb = n2 / c1;
     const double Fc = G.f0 + (double)G.kc * G.df;
-    const int ngtypes = (KIND == 2) ? 1 : 2;
-    for (int g = 1; g <= ngtypes; ++g) {
+    constexpr int NG = (KIND == 2) ? 1 : 2;
+    for (int i = tid; i < nb; i += NTP) {
+        hist[0][i] = 0;
+        hist[1][i] = 0;
+    }
+    __syncthreads();
+    auto cell_of = [&](double dt, int g, double& yc) -> int {
+        const double x = dt * ((double)g * G.df);
+        yc = (x - floor(x)) * (double)n;
+        return (int)floor(yc) - m;
+    };
+    for (int j = tid; j < N; j += NTP) {
+        const double dt = t[G.obs + j] - tref;
+        if (!isfinite(dt)) continue;  // non-finite times contribute nothing
+#pragma unroll
+        for (int g = 1; g <= NG; ++g) {
+            double yc;
+            const int U = cell_of(dt, g, yc);
+            atomicAdd(&hist[g - 1][(U & (n2 - 1)) / c1], 1);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 1; g <= NG; ++g) {
         int32_t* bk = bkt + G.bkt + (int64_t)(g - 1) * (nb + 1);
-        for (int i = tid; i < nb; i += NT) hist[i] = 0;
-        __syncthreads();
-        const double gdf = (double)g * G.df;
-        for (int j = tid; j < N; j += NT) {
-            const double dt = t[G.obs + j] - tref;
-            if (!isfinite(dt)) continue;  // non-finite times contribute nothing
-            const double x = dt * gdf;
-            const double y = (x - floor(x)) * (double)n;
-            const int U = (int)floor(y) - m;
-            atomicAdd(&hist[(U & (n2 - 1)) / c1], 1);
+        const int total = block_exclusive_scan<NTP>(hist[g - 1], nb, wsum);
+        for (int i = tid; i < nb; i += NTP) bk[i] = hist[g - 1][i];
+        if (tid == 0) bk[nb] = total;
+    }
+    __syncthreads();
+    const bool lowfix = KIND != 2 && pa.dt != nullptr;
+    for (int j = tid; j < N; j += NTP) {
+        PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
+        const double dt = p.t - tref;
+        double w, yw;
+        weights_of(p, w, yw);
+        if (lowfix) {
+            // prepared points for the low-frequency direct sums (original order)
+            pa.w[G.pts + j] = isfinite(dt) ? (float)w : 0.f;
+            pa.yw[G.pts + j] = isfinite(dt) ? (float)yw : 0.f;
+            pa.dt[G.pts + j] = isfinite(dt) ? dt : 0.0;
         }
-        __syncthreads();
-        if (tid == 0) {
-            // exclusive scan (nb <= 4096, a few microseconds at most)
-            int acc = 0;
-            for (int i = 0; i < nb; ++i) {
-                const int h = hist[i];
-                hist[i] = acc;
-                bk[i] = acc;
-                acc += h;
-            }
-            bk[nb] = acc;
-        }
-        __syncthreads();
-        for (int j = tid; j < N; j += NT) {
-            PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
-            const double dt = p.t - tref;
-            if (!isfinite(dt)) continue;
-            const double x = dt * gdf;
-            const double yc = (x - floor(x)) * (double)n;
-            const double fl = floor(yc);
-            const int U = (int)fl - m;
-            const float fr = (float)(yc - fl);
+        if (!isfinite(dt)) continue;
+#pragma unroll
+        for (int g = 1; g <= NG; ++g) {
+            double yc;
+            const int U = cell_of(dt, g, yc);
+            const float fr = (float)(yc - floor(yc));
             const float2 e = expi_rev((double)g * (Fc * dt));
-            double w, yw;
-            weights_of(p, w, yw);
-            const int pos = atomicAdd(&hist[(U & (n2 - 1)) / c1], 1);
+            const int pos = atomicAdd(&hist[g - 1][(U & (n2 - 1)) / c1], 1);
             const int64_t o = G.pts + pos;
             if (g == 1) {
                 pa.U1[o] = U;
@@ -204,27 +247,34 @@ __global__ __launch_bounds__(NT) void prep_kernel(const CurveGeom* __restrict__ 
                 pa.v2[o] = make_float2((float)w * e.x, (float)w * e.y);
             }
         }
-        __syncthreads();
     }
 }
 
 }  // namespace
 
-hipError_t launch_prep(const CurveGeom* d_geom, int ncurves, CurveState* state,
-                       const double* t, const void* a, const void* b, PointArrays pa,
-                       int32_t* bkt, const PlanParams& pp, int y_is_f32, hipStream_t s) {
-    if (ncurves <= 0) return hipSuccess;
-    dim3 grid(ncurves), block(NT);
-    if (pp.kind == 1) {
-        hipLaunchKernelGGL(prep_kernel<2>, grid, block, 0, s, d_geom, state, t, a, b, pa, bkt,
-                           pp.m, GLS_METHOD_NFFT, y_is_f32);
-    } else if (pp.input_kind == GLS_INPUT_RAW) {
-        hipLaunchKernelGGL(prep_kernel<0>, grid, block, 0, s, d_geom, state, t, a, b, pa, bkt,
-                           pp.m, pp.method, 0);
-    } else {
-        hipLaunchKernelGGL(prep_kernel<1>, grid, block, 0, s, d_geom, state, t, a, b, pa, bkt,
-                           pp.m, pp.method, 0);
+hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall,
+                       const int32_t* wide, int nwide, CurveState* state, const double* t,
+                       const void* a, const void* b, PointArrays pa, int32_t* bkt,
+                       const PlanParams& pp, int y_is_f32, hipStream_t s) {
+    const int kind = (pp.kind == 1) ? 2 : (pp.input_kind == GLS_INPUT_RAW ? 0 : 1);
+    const int method = (pp.kind == 1) ? GLS_METHOD_NFFT : pp.method;
+    const int yf = (pp.kind == 1) ? y_is_f32 : 0;
+#define GLS_PREP_LAUNCH(K, NTP, LIST, COUNT)                                                  \
+    hipLaunchKernelGGL((prep_kernel<K, NTP>), dim3(COUNT), dim3(NTP), 0, s, d_geom, LIST,    \
+                       state, t, a, b, pa, bkt, pp.m, method, yf)
+    if (nsmall > 0) {
+        if (kind == 0) GLS_PREP_LAUNCH(0, 256, small, nsmall);
+        else if (kind == 1) GLS_PREP_LAUNCH(1, 256, small, nsmall);
+        else GLS_PREP_LAUNCH(2, 256, small, nsmall);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
+    if (nwide > 0) {
+        if (kind == 0) GLS_PREP_LAUNCH(0, 1024, wide, nwide);
+        else if (kind == 1) GLS_PREP_LAUNCH(1, 1024, wide, nwide);
+        else GLS_PREP_LAUNCH(2, 1024, wide, nwide);
+    }
+#undef GLS_PREP_LAUNCH
     return hipGetLastError();
 }
 

@@ -113,7 +113,7 @@ class Plan:
     def destroy(self):
         h, self.handle = getattr(self, "handle", None), None
         if h and _lib._lib is not None:
-            _lib._lib.gls_plan_destroy(C.c_void_p(h))
+            _lib.call("gls_plan_destroy", C.c_void_p(h))
 
     def __del__(self):
         try:
@@ -134,7 +134,7 @@ class Graph:
     def destroy(self):
         h, self.handle = getattr(self, "handle", None), None
         if h and _lib._lib is not None:
-            _lib._lib.gls_graph_destroy(C.c_void_p(h))
+            _lib.call("gls_graph_destroy", C.c_void_p(h))
 
     def __del__(self):
         try:

@@ -138,7 +138,11 @@ def serve(monkeypatch):
     monkeypatch.setattr(_lib, "call", dev.call)
     monkeypatch.setattr(_lib, "init", lambda device=0: None)
     monkeypatch.setattr(srvmod.time, "sleep", lambda s: None)
-    return srvmod
+    yield srvmod
+    # release the simulated streams, events and buffers while the stand-in is
+    # still in place (their handles mean nothing to the real library)
+    import gc
+    gc.collect()
 
 
 def _requests(n, seed=3):
