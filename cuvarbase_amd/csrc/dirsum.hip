@@ -161,29 +161,35 @@ __global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __re
 // frequencies exit at once.
 constexpr int kLowPerBlock = NT / kWave;
 
-template <int OUT>
+// WIDE: one frequency per block, all NT lanes over the points (long curves)
+template <int OUT, bool WIDE>
 __global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict__ geom,
+                                                     const int32_t* __restrict__ list,
                                                      const CurveState* __restrict__ state,
                                                      PointArrays pa, PlanParams pp,
                                                      void* __restrict__ out) {
+    __shared__ double red[NT / kWave][6];
+    constexpr int FPB = WIDE ? 1 : kLowPerBlock;   // frequencies per block
+    constexpr int LANES = WIDE ? NT : kWave;       // lanes per frequency
     const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int k = blockIdx.y * kLowPerBlock + tid / kWave;
-    const CurveGeom& G = geom[blockIdx.x];
-    const CurveState& st = state[blockIdx.x];
+    const int lane = tid % LANES;
+    const int k = blockIdx.y * FPB + tid / LANES;
+    const int c = list[blockIdx.x];
+    const CurveGeom& G = geom[c];
+    const CurveState& st = state[c];
     // frequency k is low if it and every frequency before it span fewer than
     // kLowCycles cycles over the baseline (the leading run of the grid)
-    const int first = blockIdx.y * kLowPerBlock;
+    const int first = blockIdx.y * FPB;
     if (first >= min(G.nf, kLowMax)) return;
     if (fabs(G.f0 + (double)first * G.df) * st.tspan >= kLowCycles) return;
     bool active = k < min(G.nf, kLowMax);
     for (int q = first; q <= k && active; ++q)
         active = fabs(G.f0 + (double)q * G.df) * st.tspan < kLowCycles;
-    if (!active) return;   // whole waves only: k is uniform within a wave
+    if (!active) return;   // uniform per frequency: whole waves (or the block) exit
     const double f = G.f0 + (double)k * G.df;
     const int N = G.n_obs;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int j = lane; j < N; j += kWave) {
+    for (int j = lane; j < N; j += LANES) {
         const double x = f * pa.dt[G.pts + j];
         double sn, cs;
         sincospi(2.0 * (x - rint(x)), &sn, &cs);
@@ -197,6 +203,15 @@ __global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
+    if (WIDE) {
+        const int wv = tid / kWave;
+        if ((tid & (kWave - 1)) == 0)
+            for (int q = 0; q < 6; ++q) red[wv][q] = acc[q];
+        __syncthreads();
+        if (tid == 0)
+            for (int w = 1; w < NT / kWave; ++w)
+                for (int q = 0; q < 6; ++q) acc[q] += red[w][q];
+    }
     if (lane == 0) {
         const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
                                    st.ybar, pp.mode, pp.reg_c, pp.reg_s);
@@ -221,17 +236,30 @@ hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
 
 }  // namespace
 
-hipError_t launch_lowfreq(const CurveGeom* d_geom, const CurveState* state, int ncurves,
+hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsmall,
+                          const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
                           hipStream_t s) {
-    if (ncurves <= 0) return hipSuccess;
-    const dim3 grid(ncurves, (kLowMax + kLowPerBlock - 1) / kLowPerBlock);
-    if (out_f64)
-        hipLaunchKernelGGL(lowfreq_kernel<1>, grid, dim3(NT), 0, s, d_geom, state, pa, pp,
-                           power);
-    else
-        hipLaunchKernelGGL(lowfreq_kernel<0>, grid, dim3(NT), 0, s, d_geom, state, pa, pp,
-                           power);
+    if (nsmall > 0) {
+        const dim3 grid(nsmall, (kLowMax + kLowPerBlock - 1) / kLowPerBlock);
+        if (out_f64)
+            hipLaunchKernelGGL((lowfreq_kernel<1, false>), grid, dim3(NT), 0, s, d_geom, small,
+                               state, pa, pp, power);
+        else
+            hipLaunchKernelGGL((lowfreq_kernel<0, false>), grid, dim3(NT), 0, s, d_geom, small,
+                               state, pa, pp, power);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nwide > 0) {
+        const dim3 grid(nwide, kLowMax);
+        if (out_f64)
+            hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(NT), 0, s, d_geom, wide,
+                               state, pa, pp, power);
+        else
+            hipLaunchKernelGGL((lowfreq_kernel<0, true>), grid, dim3(NT), 0, s, d_geom, wide,
+                               state, pa, pp, power);
+    }
     return hipGetLastError();
 }
 

@@ -141,7 +141,8 @@ hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
 // kLowCycles (at most kLowMax of them), overwriting the NFFT powers there
 constexpr double kLowCycles = 3.0;
 constexpr int kLowMax = 256;
-hipError_t launch_lowfreq(const CurveGeom* d_geom, const CurveState* state, int ncurves,
+hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsmall,
+                          const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
                           hipStream_t s);
 hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,

@@ -469,7 +469,9 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             }
         }
         if (P->kind == 0) {
-            GLS_HIP_TRY(launch_lowfreq(d_geom, state, P->ncurves, pa, pp, out, out_f64, s));
+            GLS_HIP_TRY(launch_lowfreq(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
+                                       P->ncurves - P->nprep_small, state, pa, pp, out,
+                                       out_f64, s));
             GLS_HIP_TRY(mark());
         }
     } else if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) {

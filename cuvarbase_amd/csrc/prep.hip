@@ -105,6 +105,7 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
 
     // ---- phase A: reductions ------------------------------------------------
     double tmin = INFINITY, tmax = -INFINITY, sy = 0.0, cnt = 0.0, s0 = 0.0;
+#pragma unroll 4
     for (int j = tid; j < N; j += NTP) {
         PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
         tmin = fmin(tmin, p.t);
@@ -127,13 +128,15 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
         ymean = sy / cnt;
         s0inv = 1.0 / s0;
         double s1 = 0.0;
-        for (int j = tid; j < N; j += NTP) {
+    #pragma unroll 4
+    for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             s1 += (p.yv - ymean) / (p.wv * p.wv);
         }
         ybar = block_sum<NTP>(s1, red) * s0inv;
         double s2 = 0.0;
-        for (int j = tid; j < N; j += NTP) {
+    #pragma unroll 4
+    for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             const double d = p.yv - ymean - ybar;
             s2 += d * d / (p.wv * p.wv);
@@ -165,7 +168,8 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
     };
 
     if (method == GLS_METHOD_DIRECT || method == GLS_METHOD_DIRECT_CUSTOM) {
-        for (int j = tid; j < N; j += NTP) {
+    #pragma unroll 4
+    for (int j = tid; j < N; j += NTP) {
             PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
             double w, yw;
             weights_of(p, w, yw);
@@ -196,6 +200,7 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
         yc = (x - floor(x)) * (double)n;
         return (int)floor(yc) - m;
     };
+#pragma unroll 4
     for (int j = tid; j < N; j += NTP) {
         const double dt = t[G.obs + j] - tref;
         if (!isfinite(dt)) continue;  // non-finite times contribute nothing
@@ -216,6 +221,7 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
     }
     __syncthreads();
     const bool lowfix = KIND != 2 && pa.dt != nullptr;
+#pragma unroll 4
     for (int j = tid; j < N; j += NTP) {
         PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
         const double dt = p.t - tref;
