@@ -269,7 +269,7 @@ def main():
 
         arr, sdata, sfreqs = synthetic.stream_requests(1.0, nreq=args.stream_requests)
         srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m,
-                        zero_copy=True, **win)
+                        zero_copy=True, max_batch_points=8_000_000, **win)
         serving = {"requests_per_rate": args.stream_requests, "hbm_budget_gb": args.hbm_budget_gb,
                    "rates": {}}
         for rate in (100.0, 1000.0, 10000.0, float("inf")):

@@ -177,7 +177,7 @@ class _Slot:
     def __init__(self):
         self.stream = Stream()
         self.event = Event()
-        self.h_in = None      # pinned (3, max_obs) f64
+        self.h_in = None      # pinned f64: t, y, dy of a batch, one after another
         self.h_out = None     # pinned f32
         self.h_freq = None    # pinned f64 (custom frequency lists)
         self.h_idx = None     # pinned i32 (top-k indices)
@@ -188,9 +188,8 @@ class _Slot:
         def grow(cur, need):
             return max(need, 1024, 0 if cur is None else 2 * len(cur))
 
-        if self.h_in is None or self.h_in.shape[1] < nobs:
-            self.h_in = pinned_empty((3, grow(None if self.h_in is None else self.h_in[0],
-                                              nobs)), np.float64)
+        if self.h_in is None or self.h_in.size < 3 * nobs:
+            self.h_in = pinned_empty(grow(self.h_in, 3 * nobs), np.float64)
         if nout and (self.h_out is None or self.h_out.size < nout):
             self.h_out = pinned_empty(grow(self.h_out, nout), np.float32)
         if nfreq and (self.h_freq is None or self.h_freq.size < nfreq):
@@ -221,6 +220,9 @@ class GLSServer:
     hbm_budget : bytes of device memory the server may use (resident arena)
     slots : number of batches that may be in flight at once (one stream each)
     max_batch_evals : cap on curve x frequency evaluations per batch
+    max_batch_points : cap on observations per batch; each slot's page-locked
+        input staging (24 B per point) is allocated for it at start-up (a
+        single larger request still runs, growing its slot's staging)
     sigma, m, window, window_beta : NFFT parameters (default: tuned for
         |dP| <= ``tol``)
     zero_copy : results are views into a page-locked result arena of
@@ -232,7 +234,8 @@ class GLSServer:
 
     def __init__(self, hbm_budget=16e9, slots=3, max_batch_evals=3e8, max_batch_curves=8192,
                  sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN, zero_copy=False,
-                 result_buffer_bytes=None, window=None, window_beta=0.0):
+                 result_buffer_bytes=None, window=None, window_beta=0.0,
+                 max_batch_points=1 << 20):
         _lib.init(0)
         cfg = choose_nfft_config(tol)
         if sigma is not None or m is not None or window is not None:
@@ -247,6 +250,7 @@ class GLSServer:
         self.mode = mode
         self.max_batch_evals = int(max_batch_evals)
         self.max_batch_curves = int(max_batch_curves)
+        self.max_batch_points = max(int(max_batch_points), 1)
         self.arena = DeviceArena(hbm_budget)
         self.zero_copy = bool(zero_copy)
         self.host_arena = None
@@ -254,6 +258,8 @@ class GLSServer:
             nb = result_buffer_bytes or 4 * self.max_batch_evals * (slots + 1)
             self.host_arena = HostArena(int(nb))
         self.slots = [_Slot() for _ in range(slots)]
+        for slot in self.slots:
+            slot.staging(self.max_batch_points, 0)
         self.queue = deque()
         self.completed = []
         self.batches_launched = 0
@@ -312,16 +318,18 @@ class GLSServer:
         if not self.queue:
             return []
         cls = self._class(self.queue[0])
-        reqs, rest, ev = [], [], 0
+        reqs, rest, ev, pts = [], [], 0, 0
         while self.queue and len(reqs) < self.max_batch_curves:
             r = self.queue[0]
             if self._class(r) != cls:
                 rest.append(self.queue.popleft())
                 continue
-            if reqs and ev + r.nf > self.max_batch_evals:
+            if reqs and (ev + r.nf > self.max_batch_evals
+                         or pts + len(r.t) > self.max_batch_points):
                 break
             reqs.append(self.queue.popleft())
             ev += r.nf
+            pts += len(r.t)
         self.queue.extendleft(reversed(rest))
         return reqs
 
@@ -363,21 +371,22 @@ class GLSServer:
                 return False
         h_in, h_out = slot.staging(obs, 0 if self.zero_copy else (nreq * k if k else out),
                                    out if custom else 0, 0 if self.zero_copy else nreq * k)
+        # rows t, y, dy of the batch back to back, as the device block holds
+        # them: one copy
+        rows = h_in[:3 * obs].reshape(3, obs)
         o = 0
         for r in reqs:
             n = len(r.t)
-            h_in[0, o:o + n] = r.t
-            h_in[1, o:o + n] = r.y
-            h_in[2, o:o + n] = r.dy
+            rows[0, o:o + n] = r.t
+            rows[1, o:o + n] = r.y
+            rows[2, o:o + n] = r.dy
             o += n
         s = slot.stream.handle
         p_in = self.arena.ptr(blocks[0])
         p_pow = self.arena.ptr(blocks[1])
         p_ws = self.arena.ptr(blocks[2])
         row = 8 * max(obs, 1)
-        for q in range(3):
-            _lib.call("gls_memcpy_h2d_async", p_in + q * row, h_in[q].ctypes.data,
-                      C.c_size_t(8 * obs), s)
+        _lib.call("gls_memcpy_h2d_async", p_in, h_in.ctypes.data, C.c_size_t(24 * obs), s)
         if custom:
             for r, o in zip(reqs, offsets):
                 slot.h_freq[o:o + r.nf] = r.freqs

@@ -138,6 +138,7 @@ def serve(monkeypatch):
     monkeypatch.setattr(_lib, "call", dev.call)
     monkeypatch.setattr(_lib, "init", lambda device=0: None)
     monkeypatch.setattr(srvmod.time, "sleep", lambda s: None)
+    monkeypatch.setattr(srvmod, "_test_device", dev, raising=False)
     yield srvmod
     # release the simulated streams, events and buffers while the stand-in is
     # still in place (their handles mean nothing to the real library)
@@ -214,3 +215,19 @@ def test_unreleased_results_block_admission(serve):
         srv.drain()
     st["requests_obj"][0].release()
     assert srv.host_arena.used == 0
+
+
+def test_batches_respect_point_cap(serve):
+    data, freqs = _requests(20, seed=6)
+    cap = 150
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=2, max_batch_evals=10**6,
+                          max_batch_points=cap)
+    assert all(s.h_in.size >= 3 * cap for s in srv.slots)   # staging sized up front
+    st = srv.serve_stream(np.zeros(len(data)), data, freqs)
+    plans = serve._test_device.plans.values()
+    assert len(plans) == st["batches"] > 1
+    for p in plans:
+        pts = sum(c[1] for c in p["curves"])
+        assert pts <= cap or len(p["curves"]) == 1
+    for r, d, f in zip(st["requests_obj"], data, freqs):
+        np.testing.assert_allclose(r.power, og.gls_direct(*d, f), atol=1e-6)
