@@ -57,7 +57,7 @@ struct CurveGeom {
     int32_t p2;        // log2 of pass-2 row length n2
     int32_t c1;        // pass-1 columns per tile
     int32_t kc;        // K/2, band centre index
-    int32_t pad;
+    int32_t ci;        // index of the curve in the plan
 };
 
 // written by the prep kernel
@@ -119,19 +119,18 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        const double* t, const void* a, const void* b,
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
                        int y_is_f32, hipStream_t s);
-hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves,
-                        int ncurves, int ngrids, int p2, PointArrays pa,
-                        const int32_t* bkt, float2* grids, const PlanParams& pp,
-                        hipStream_t s);
-hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom,
-                             const CurveState* state, const int32_t* curves,
-                             int ncurves, const float2* grids,
+// pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
+// curves, one after another (a block reads its curve's entry directly)
+hipError_t launch_pass1(int n1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
+                        PointArrays pa, const int32_t* bkt, float2* grids,
+                        const PlanParams& pp, hipStream_t s);
+hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* bgeom,
+                             const CurveState* state, int ncurves, const float2* grids,
                              const PlanParams& pp, void* power, int out_f64,
                              hipStream_t s);
-hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom,
-                             const CurveState* state, const int32_t* curves,
-                             int ncurves, const float2* grids, const PlanParams& pp,
-                             float2* out, hipStream_t s);
+hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* bgeom,
+                             const CurveState* state, int ncurves, const float2* grids,
+                             const PlanParams& pp, float2* out, hipStream_t s);
 hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const int32_t* tile_curve, const int32_t* tile_first,
                          int ntiles, int rf, PointArrays pa,

@@ -27,8 +27,7 @@ constexpr int NT1 = 256;
 // pass 1: spread + column FFT + inter-pass twiddle
 // ---------------------------------------------------------------------------
 template <int N1>
-__global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict__ geom,
-                                                    const int32_t* __restrict__ curves,
+__global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict__ bgeom,
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
                                                     float2* __restrict__ grids, int m,
@@ -46,8 +45,7 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     const int bid = xcd_logical_block(blockIdx.x, gridDim.x);
     const int jb = bid % nb;
     const int gi = (bid / nb) % ngrids;
-    const int ci = curves[bid / (nb * ngrids)];
-    const CurveGeom& G = geom[ci];
+    const CurveGeom& G = bgeom[bid / (nb * ngrids)];
     const int n = G.n;
     const int a = jb * C1;
 
@@ -170,9 +168,8 @@ constexpr int pass2_threads() { return OUT == 2 ? 128 : (P2 >= 9 ? 384 : 192); }
 
 template <int P2, int OUT>
 __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
-    const CurveGeom* __restrict__ geom, const CurveState* __restrict__ state,
-    const int32_t* __restrict__ curves, int n1, const float2* __restrict__ grids,
-    PlanParams pp, void* __restrict__ out) {
+    const CurveGeom* __restrict__ bgeom, const CurveState* __restrict__ state, int n1,
+    const float2* __restrict__ grids, PlanParams pp, void* __restrict__ out) {
     constexpr int NT = pass2_threads<P2, OUT>();
     constexpr int N2 = 1 << P2;
     constexpr int CR = pass2_rows(P2);
@@ -185,8 +182,8 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const int nrb = (n1 + CR - 1) / CR;   // the last row block may be partial
     const int bid = xcd_logical_block(blockIdx.x, gridDim.x);
     const int rb = bid % nrb;
-    const int ci = curves[bid / nrb];
-    const CurveGeom& G = geom[ci];
+    const CurveGeom& G = bgeom[bid / nrb];
+    const int ci = G.ci;
     const int n = G.n;
     const int r0 = rb * CR;
 
@@ -248,76 +245,71 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
 }
 
 template <int N1>
-hipError_t pass1_dispatch(const CurveGeom* d_geom, const int32_t* curves, int ncurves,
-                          int ngrids, int p2, PointArrays pa, const int32_t* bkt,
-                          float2* grids, const PlanParams& pp, hipStream_t s) {
+hipError_t pass1_dispatch(const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
+                          PointArrays pa, const int32_t* bkt, float2* grids,
+                          const PlanParams& pp, hipStream_t s) {
     constexpr int C1 = pass1_cols(N1);
     const int nb = (1 << p2) / C1;
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, d_geom,
-                       curves, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta);
+    hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, bgeom,
+                       ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta);
     return hipGetLastError();
 }
 
 template <int P2, int OUT>
-hipError_t pass2_dispatch(int n1, const CurveGeom* d_geom, const CurveState* state,
-                          const int32_t* curves, int ncurves, const float2* grids,
-                          const PlanParams& pp, void* out, hipStream_t s) {
+hipError_t pass2_dispatch(int n1, const CurveGeom* bgeom, const CurveState* state, int ncurves,
+                          const float2* grids, const PlanParams& pp, void* out, hipStream_t s) {
     constexpr int CR = pass2_rows(P2);
     const int64_t nblocks = (int64_t)ncurves * ((n1 + CR - 1) / CR);
     if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((pass2_kernel<P2, OUT>), dim3((unsigned)nblocks),
-                       dim3(pass2_threads<P2, OUT>()), 0, s, d_geom, state, curves, n1, grids, pp,
-                       out);
+                       dim3(pass2_threads<P2, OUT>()), 0, s, bgeom, state, n1, grids, pp, out);
     return hipGetLastError();
 }
 
 template <int OUT>
-hipError_t pass2_switch(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
-                        const int32_t* curves, int ncurves, const float2* grids,
-                        const PlanParams& pp, void* out, hipStream_t s) {
+hipError_t pass2_switch(int p2, int n1, const CurveGeom* bgeom, const CurveState* state,
+                        int ncurves, const float2* grids, const PlanParams& pp, void* out,
+                        hipStream_t s) {
+#define GLS_P2_CASE(v) \
+    case v: return pass2_dispatch<v, OUT>(n1, bgeom, state, ncurves, grids, pp, out, s);
     switch (p2) {
-        case 6: return pass2_dispatch<6, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 7: return pass2_dispatch<7, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 8: return pass2_dispatch<8, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 9: return pass2_dispatch<9, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 10: return pass2_dispatch<10, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 11: return pass2_dispatch<11, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
-        case 12: return pass2_dispatch<12, OUT>(n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+        GLS_P2_CASE(6) GLS_P2_CASE(7) GLS_P2_CASE(8) GLS_P2_CASE(9) GLS_P2_CASE(10)
+        GLS_P2_CASE(11) GLS_P2_CASE(12)
         default: return hipErrorInvalidValue;
     }
+#undef GLS_P2_CASE
 }
 
 }  // namespace
 
-hipError_t launch_pass1(int n1, const CurveGeom* d_geom, const int32_t* curves, int ncurves,
-                        int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
+hipError_t launch_pass1(int n1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
+                        PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
     switch (n1) {
 #define GLS_PASS1_CASE(v) \
-    case v: return pass1_dispatch<v>(d_geom, curves, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
+    case v: return pass1_dispatch<v>(bgeom, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
         GLS_PASS1_SIZES(GLS_PASS1_CASE)
 #undef GLS_PASS1_CASE
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
-                             const int32_t* curves, int ncurves, const float2* grids,
-                             const PlanParams& pp, void* power, int out_f64, hipStream_t s) {
+hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* bgeom, const CurveState* state,
+                             int ncurves, const float2* grids, const PlanParams& pp,
+                             void* power, int out_f64, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
-    if (out_f64)
-        return pass2_switch<1>(p2, n1, d_geom, state, curves, ncurves, grids, pp, power, s);
-    return pass2_switch<0>(p2, n1, d_geom, state, curves, ncurves, grids, pp, power, s);
+    if (out_f64) return pass2_switch<1>(p2, n1, bgeom, state, ncurves, grids, pp, power, s);
+    return pass2_switch<0>(p2, n1, bgeom, state, ncurves, grids, pp, power, s);
 }
 
-hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* d_geom, const CurveState* state,
-                             const int32_t* curves, int ncurves, const float2* grids,
-                             const PlanParams& pp, float2* out, hipStream_t s) {
+hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* bgeom, const CurveState* state,
+                             int ncurves, const float2* grids, const PlanParams& pp,
+                             float2* out, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
-    return pass2_switch<2>(p2, n1, d_geom, state, curves, ncurves, grids, pp, out, s);
+    return pass2_switch<2>(p2, n1, bgeom, state, ncurves, grids, pp, out, s);
 }
 
 }  // namespace gls

@@ -43,7 +43,7 @@ struct gls_plan {
     int rf = 1;
     // host image of the descriptors uploaded into workspace[0 .. meta.size())
     std::vector<char> meta;
-    size_t off_geom = 0, off_bcurves = 0, off_tiles = 0, off_estab = 0, off_prep = 0;
+    size_t off_geom = 0, off_bgeom = 0, off_tiles = 0, off_estab = 0, off_prep = 0;
     std::vector<int32_t> prep_list;   // curves prepared by 256-thread blocks, then wide ones
     int nprep_small = 0;
     std::vector<float> estab;      // ES deconvolution table (empty for the Gaussian)
@@ -352,7 +352,7 @@ void reserve_meta(gls_plan* P, Layout& L, size_t nbcurves, size_t ntiles_ints) {
         if (P->geom[i].n_obs > kWidePrep) P->prep_list.push_back(i);
     P->off_geom = L.take(sizeof(CurveGeom) * P->geom.size());
     P->off_prep = L.take(sizeof(int32_t) * P->prep_list.size());
-    P->off_bcurves = L.take(sizeof(int32_t) * nbcurves);
+    P->off_bgeom = L.take(sizeof(CurveGeom) * nbcurves);
     P->off_estab = L.take(sizeof(float) * P->estab.size());
     P->off_tiles = L.take(sizeof(int32_t) * ntiles_ints);
 }
@@ -369,9 +369,10 @@ void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
     if (!P->geom.empty())
         std::memcpy(P->meta.data() + P->off_geom, P->geom.data(),
                     sizeof(CurveGeom) * P->geom.size());
-    if (!bcurves.empty())
-        std::memcpy(P->meta.data() + P->off_bcurves, bcurves.data(),
-                    sizeof(int32_t) * bcurves.size());
+    // the geometry again in bucket order, for the pass kernels
+    for (size_t i = 0; i < bcurves.size(); ++i)
+        std::memcpy(P->meta.data() + P->off_bgeom + i * sizeof(CurveGeom),
+                    &P->geom[bcurves[i]], sizeof(CurveGeom));
     if (!tiles.empty())
         std::memcpy(P->meta.data() + P->off_tiles, tiles.data(), sizeof(int32_t) * tiles.size());
 }
@@ -409,7 +410,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         GLS_HIP_TRY(hipMemcpyAsync(base, P->meta.data(), P->meta.size(), hipMemcpyHostToDevice,
                                    s));
     const CurveGeom* d_geom = (const CurveGeom*)(base + P->off_geom);
-    const int32_t* d_bcurves = (const int32_t*)(base + P->off_bcurves);
+    const CurveGeom* d_bgeom = (const CurveGeom*)(base + P->off_bgeom);
     const int32_t* d_tiles = (const int32_t*)(base + P->off_tiles);
     CurveState* state = (CurveState*)(base + P->off_state);
     PointArrays pa = point_arrays(P, ws);
@@ -450,16 +451,15 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
             const auto& B = P->buckets[bi];
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
-            const int32_t* curves = d_bcurves + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, d_geom, curves, B.count, ngrids, B.p2, pa, bkt, grids,
-                                     pp, ls));
+            const CurveGeom* bgeom = d_bgeom + B.first;
+            GLS_HIP_TRY(launch_pass1(B.n1, bgeom, B.count, ngrids, B.p2, pa, bkt, grids, pp, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
-                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              pp, (float2*)out, ls));
+                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, bgeom, state, B.count, grids, pp,
+                                              (float2*)out, ls));
             else
-                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, d_geom, state, curves, B.count, grids,
-                                              pp, out, out_f64, ls));
+                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, bgeom, state, B.count, grids, pp, out,
+                                              out_f64, ls));
             GLS_HIP_TRY(mark());
         }
         if (fork) {
@@ -573,6 +573,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         }
         CurveGeom& G = P->geom[i];
         G = CurveGeom{};
+        G.ci = i;
         G.obs = c.obs_offset;
         G.out = c.out_offset;
         G.pts = pts;
@@ -655,6 +656,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
         }
         CurveGeom& G = P->geom[i];
         G = CurveGeom{};
+        G.ci = i;
         G.obs = c.obs_offset;
         G.out = c.out_offset;
         G.pts = pts;
