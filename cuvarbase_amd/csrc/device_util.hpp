@@ -132,4 +132,46 @@ __device__ __forceinline__ double gls_power(double C, double S, double C2, doubl
     return P;
 }
 
+// The same power with the sums and their combinations in FP32, as the NFFT
+// assembly runs: the determinant CC SS - CS^2 and the numerator are formed in
+// FP64 from the FP32 terms (the products are exact there, so their
+// cancellation costs no digits beyond those already in CC, SS, CS), and the
+// quotient in FP32.  Frequencies with f (t_max - t_min) < 3, where CC, SS and
+// CS cancel themselves, are recomputed by the FP64 direct sums.
+__device__ __forceinline__ float gls_power_f32(float C, float S, float C2, float S2, float YC,
+                                               float YS, float YY, float ybar, int mode,
+                                               float l0, float l1) {
+    float CC, SS, CS, bc, bs, den;
+    if (mode == GLS_MODE_FLOATING_MEAN) {
+        CC = 0.5f * (1.0f + C2) - C * C + l0;
+        SS = 0.5f * (1.0f - C2) - S * S + l1;
+        CS = 0.5f * S2 - C * S;
+        bc = YC;
+        bs = YS;
+        den = YY;
+    } else if (mode == GLS_MODE_STANDARD) {
+        CC = 0.5f * (1.0f + C2) + l0;
+        SS = 0.5f * (1.0f - C2) + l1;
+        CS = 0.5f * S2;
+        bc = YC + ybar * C;
+        bs = YS + ybar * S;
+        den = YY + ybar * ybar;
+    } else if (mode == GLS_MODE_WINDOW) {
+        CC = 0.5f * (1.0f + C2);
+        SS = 0.5f * (1.0f - C2);
+        CS = 0.5f * S2;
+        bc = 2.0f * C;
+        bs = 2.0f * S;
+        den = 1.0f;
+    } else {
+        return -1.0f;
+    }
+    const double cc = CC, ss = SS, cs = CS, xc = bc, xs = bs;
+    const double D = cc * ss - cs * cs;
+    const double N = ss * xc * xc + cc * xs * xs - 2.0 * cs * xc * xs;
+    const float P = (float)N * __builtin_amdgcn_rcpf(den * (float)D);
+    if (!isfinite(P) || P < 0.0f) return -1.0f;
+    return P;
+}
+
 }  // namespace gls

@@ -203,6 +203,8 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const float bpi2 = (float)(G.b * M_PI * M_PI);
     const float ninv = 1.0f / (float)n;
     const CurveState st = state[ci];
+    const float yy32 = (float)st.yy, ybar32 = (float)st.ybar;
+    const float rc32 = (float)pp.reg_c, rs32 = (float)pp.reg_s;
     for (int i = tid; i < N2 * CR; i += NT) {
         const int r = i % CR;
         const int k2 = i / CR;
@@ -235,11 +237,17 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
             const float2 syw = tile[lds_at<BS>(k2, r)];
             const float2 w1 = tile[lds_at<BS>(k2, CR + r)];
             const float2 w2 = tile[lds_at<BS>(k2, 2 * CR + r)];
-            const double d = (double)dc;
-            const double P = gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x,
-                                       d * syw.y, st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
-            if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
-            else ((double*)out)[G.out + k] = P;
+            if (OUT == 0) {
+                ((float*)out)[G.out + k] =
+                    gls_power_f32(dc * w1.x, dc * w1.y, dc * w2.x, dc * w2.y, dc * syw.x,
+                                  dc * syw.y, yy32, ybar32, pp.mode, rc32, rs32);
+            } else {
+                // FP64 power output: the assembly in FP64 as well
+                const double d = (double)dc;
+                ((double*)out)[G.out + k] =
+                    gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x, d * syw.y,
+                              st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+            }
         }
     }
 }

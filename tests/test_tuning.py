@@ -2,7 +2,8 @@
 
 The emulation restates the kernels' precision on the CPU: points spread in
 FP32, a complex64 FFT (scipy.fft computes single precision natively), FP32
-deconvolution and FP64 assembly.  The FP64 oracle NFFT at sigma=4, m=14 is the
+deconvolution and FP32 assembly with the determinant and numerator formed in
+FP64 from the FP32 terms (gls_power_f32).  The FP64 oracle NFFT at sigma=4, m=14 is the
 reference.  This pins the tuner's error model: the chosen parameters meet the
 1e-4 target with margin, and the FP32 term that rules out sharp band edges is
 real (sigma = 1.2, m = 18 is accurate in FP64 but useless in FP32).
@@ -53,6 +54,22 @@ def _nfft_fp32(t, v, f0, df, K, n, m, b, tref, es_beta=None):
     return (G[lp % n] * dc).astype(np.complex128)
 
 
+def _power_fp32(C, S, C2, S2, YC, YS, YY):
+    """Floating-mean power as the NFFT assembly forms it: CC, SS, CS in FP32,
+    D = CC SS - CS^2 and the numerator in FP64 from those, quotient in FP32."""
+    f = np.float32
+    C, S, C2, S2, YC, YS = (np.asarray(v).astype(f) for v in (C, S, C2, S2, YC, YS))
+    CC = (f(0.5) * (f(1) + C2) - C * C).astype(f)
+    SS = (f(0.5) * (f(1) - C2) - S * S).astype(f)
+    CS = (f(0.5) * S2 - C * S).astype(f)
+    cc, ss, cs, xc, xs = (v.astype(np.float64) for v in (CC, SS, CS, YC, YS))
+    D = cc * ss - cs * cs
+    N = ss * xc * xc + cc * xs * xs - 2.0 * cs * xc * xs
+    with np.errstate(divide="ignore", invalid="ignore"):
+        P = (N.astype(f) / (f(YY) * D.astype(f))).astype(np.float64)
+    return np.where(np.isfinite(P) & (P >= 0), P, -1.0)
+
+
 def _gls_fp32(t, y, dy, freqs, sigma, m, es_beta=None, lowfix=False):
     t0, y0, dy0 = t, y, dy
     t, y, dy = og.normalize_light_curve(t, y, dy)
@@ -64,7 +81,7 @@ def _gls_fp32(t, y, dy, freqs, sigma, m, es_beta=None, lowfix=False):
     a = _nfft_fp32(t, yw, f0, df, K, n, m, b, tref, es_beta)
     c = _nfft_fp32(t, w, f0, df, K, n, m, b, tref, es_beta)
     d = _nfft_fp32(t, w, 2 * f0, 2 * df, K, n, m, b, tref, es_beta)
-    P = og.power_from_sums(c.real, c.imag, d.real, d.imag, a.real, a.imag, YY, ybar)
+    P = _power_fp32(c.real, c.imag, d.real, d.imag, a.real, a.imag, YY)
     if lowfix:
         # the library's FP64 direct sums at frequencies with f (t_max - t_min) < 3
         low = np.flatnonzero(np.abs(freqs) * np.ptp(t) < 3.0)[:256]
