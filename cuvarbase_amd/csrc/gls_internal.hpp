@@ -58,6 +58,13 @@ struct CurveGeom {
     int32_t c1;        // pass-1 columns per tile
     int32_t kc;        // K/2, band centre index
     int32_t ci;        // index of the curve in the plan
+    // per-curve constants of the pass kernels, computed on the host (a
+    // division or square root in every thread would cost tens of VALU
+    // instructions per block)
+    float g_binv;      // log2(e) / b        (Gaussian spreading weight)
+    float g_norm;      // 1 / sqrt(pi b)
+    float g_bpi2;      // b pi^2             (Gaussian deconvolution)
+    float ninv;        // 1 / n (FP32)
 };
 
 // written by the prep kernel

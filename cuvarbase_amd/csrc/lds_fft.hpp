@@ -198,6 +198,15 @@ __device__ __forceinline__ float2 expi_ratio(int64_t q, int64_t n) {
     sincospif((float)(2 * r) / (float)n, &s, &c);
     return make_float2(c, s);
 }
+// the same for 0 <= q < 2^31 and 0 < n < 2^30, with a 32-bit reduction (the
+// 64-bit remainder by a run-time n is a long instruction sequence)
+__device__ __forceinline__ float2 expi_ratio32(uint32_t q, uint32_t n) {
+    const uint32_t r = q % n;
+    const int rs = (2 * r >= n) ? (int)r - (int)n : (int)r;
+    float s, c;
+    sincospif((float)(2 * rs) / (float)n, &s, &c);
+    return make_float2(c, s);
+}
 
 // two-level table of exp(2 pi i x / L), x < L: hi[x >> 6] * lo[x & 63]
 template <int L>
@@ -207,11 +216,35 @@ struct TwTable {
     float2 lo[64];
 };
 
+// The same table as compile-time constants (cos, sin of 2 pi x / L for the
+// lo entries x = 0..63, then the hi entries x = 64 h), evaluated by the
+// compiler in double precision and rounded once to float; blocks copy it into
+// LDS instead of evaluating a sine and cosine per entry.
+template <int L>
+struct TwConst {
+    static constexpr int N = 64 + TwTable<L>::NH;
+    float c[N];
+    float s[N];
+    constexpr TwConst() : c(), s() {
+        for (int i = 0; i < N; ++i) {
+            long long x = (i < 64 ? (long long)i : (long long)(i - 64) * 64) % L;
+            if (2 * x >= L) x -= L;
+            const double a = 2.0 * kPiC * (double)x / L;
+            c[i] = (float)taylor_cos(a);
+            s[i] = (float)taylor_sin(a);
+        }
+    }
+};
+template <int L>
+__device__ constexpr TwConst<L> kTwConst{};
+
 template <int L, int NT>
 __device__ __forceinline__ void build_twiddles(TwTable<L>& T, int tid) {
-    for (int i = tid; i < 64 + TwTable<L>::NH; i += NT) {
-        if (i < 64) T.lo[i] = expi_ratio(i, L);
-        else T.hi[i - 64] = expi_ratio((int64_t)(i - 64) * 64, L);
+    const TwConst<L>& C = kTwConst<L>;
+    for (int i = tid; i < TwConst<L>::N; i += NT) {
+        const float2 v = make_float2(C.c[i], C.s[i]);
+        if (i < 64) T.lo[i] = v;
+        else T.hi[i - 64] = v;
     }
 }
 

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
                                                     float2* __restrict__ grids, int m,
-                                                    int window, float es_beta) {
+                                                    int window, float es_beta, float ainv) {
     constexpr int C1 = pass1_cols(N1);
     constexpr int BS = lds_stride(C1);
     constexpr int NH = (N1 + 63) / 64;
@@ -97,9 +97,10 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     for (int i = tid; i < C1 * (NH + 64); i += NT1) {
         const int jj = i / (NH + 64);
         const int r = i - jj * (NH + 64);
-        const int64_t u2 = a + jj;
-        if (r < 64) elo[jj * 64 + r] = expi_ratio(u2 * r, n);
-        else ehi[jj * NH + (r - 64)] = expi_ratio(u2 * 64 * (r - 64), n);
+        const int u2 = a + jj;
+        // u2 < 2^12, r < 64, r - 64 < 2^6: the products stay below 2^24
+        if (r < 64) elo[jj * 64 + r] = expi_ratio32((uint32_t)(u2 * r), n);
+        else ehi[jj * NH + (r - 64)] = expi_ratio32((uint32_t)(u2 * 64 * (r - 64)), n);
     }
     __syncthreads();
 
@@ -107,9 +108,8 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     // Gaussian exp(-d^2/b)/sqrt(pi b), or ES exp(beta (sqrt(1 - z^2) - 1)),
     // z = d/(m + 1/2), zero for |z| >= 1
     const bool es = window == GLS_WINDOW_ES;
-    const float binv = (float)(1.4426950408889634 / G.b);   // log2(e)/b
-    const float norm = (float)(1.0 / sqrt(M_PI * G.b));
-    const float ainv = 1.0f / ((float)m + 0.5f);
+    const float binv = G.g_binv;   // log2(e)/b
+    const float norm = G.g_norm;
     const float bl2 = es_beta * 1.4426950408889634f;          // beta log2(e)
     const int two_m = 2 * m;
     auto spread = [&](int U, float fr, float2 v) {
@@ -200,8 +200,8 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
 
     const int K = G.nf;
     const int kc = G.kc;
-    const float bpi2 = (float)(G.b * M_PI * M_PI);
-    const float ninv = 1.0f / (float)n;
+    const float bpi2 = G.g_bpi2;
+    const float ninv = G.ninv;
     const CurveState st = state[ci];
     const float yy32 = (float)st.yy, ybar32 = (float)st.ybar;
     const float rc32 = (float)pp.reg_c, rs32 = (float)pp.reg_s;
@@ -261,7 +261,8 @@ hipError_t pass1_dispatch(const CurveGeom* bgeom, int ncurves, int ngrids, int p
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, bgeom,
-                       ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta);
+                       ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
+                       1.0f / ((float)pp.m + 0.5f));
     return hipGetLastError();
 }
 

@@ -153,6 +153,10 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         G.c1 = c1;
         G.kc = G.nf / 2;
         G.b = gauss_b((double)n / (double)G.nf, m);
+        G.g_binv = (float)(1.4426950408889634 / G.b);
+        G.g_norm = (float)(1.0 / std::sqrt(M_PI * G.b));
+        G.g_bpi2 = (float)(G.b * M_PI * M_PI);
+        G.ninv = 1.0f / (float)n;
         G.bkt = bkt_ints;
         bkt_ints += 2 * (n2 / c1 + 1);
         by_size[{n1, p2}].push_back(i);
