@@ -35,7 +35,9 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     constexpr int C1 = pass1_cols(N1);
     constexpr int BS = lds_stride(C1);
     constexpr int NH = (N1 + 63) / 64;
-    __shared__ float2 tile[lds_size(N1, BS)];
+    // (rounded up to whole 16-byte pairs so that it is cleared with 128-bit stores)
+    constexpr int TL = (lds_size(N1, BS) + 1) & ~1;
+    __shared__ __align__(16) float2 tile[TL];
     __shared__ TwTable<N1> T;
     __shared__ float2 ehi[C1 * NH];   // exp(2 pi i u2 64 kh / n)
     __shared__ float2 elo[C1 * 64];   // exp(2 pi i u2 kl / n)
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
         }
     }
 
-    for (int i = tid; i < lds_size(N1, BS); i += NT1) tile[i] = make_float2(0.f, 0.f);
+    for (int i = tid; i < TL / 2; i += NT1)
+        reinterpret_cast<float4*>(tile)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     build_twiddles<N1, NT1>(T, tid);
     for (int i = tid; i < C1 * (NH + 64); i += NT1) {
         const int jj = i / (NH + 64);
