@@ -95,6 +95,28 @@ __device__ __forceinline__ float2 expi_rev(double r) {
     return make_float2(c, s);
 }
 
+// Buffer access to one curve's grid region (at most 3 grids of 2^24
+// complex64 cells, under 2^32 bytes): the base address and size live in a
+// scalar resource, every access takes a 32-bit byte offset, so the per-access
+// address arithmetic is one 32-bit add rather than a 64-bit multiply-add, and
+// offsets past the region read zero / drop the store.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const void* base, uint32_t bytes) {
+    // 4th descriptor word: raw 32-bit access (the value gfx9 raw buffer
+    // accesses use)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+typedef float gls_v2f __attribute__((ext_vector_type(2)));
+typedef unsigned int gls_v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 grid_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const gls_v2f v = __builtin_bit_cast(gls_v2f, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void grid_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v) {
+    const gls_v2f x = {v.x, v.y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, off, 0, 0);
+}
+
 // GLS power from the six sums, closed form b^T A^-1 b / den in FP64
 // (algebraically equal to lspow / lspow_flmean / lspow0 of
 // cuvarbase/kernels/lomb.cu:36-141; invalid -> -1 as lomb.cu:79-80).

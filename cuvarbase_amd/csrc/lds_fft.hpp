@@ -268,6 +268,23 @@ __device__ __forceinline__ int lds_at(int i, int b) {
 // LDS elements needed for L rows of stride BS (largest lds_at index + 1)
 constexpr int lds_size(int L, int BS) { return L * BS + ((L * BS) >> 4) + 1; }
 
+// Element i0 + q*step of sequence b, q = 0, 1, ...: when step*BS is a multiple
+// of 16 the pad slots grow linearly with q, so the places are
+// lds_at(i0, b) + q*(step*BS*17/16) -- compile-time offsets from one base,
+// which the LDS instructions take as immediates instead of a shift, mask and
+// add per element.
+template <int BS, int STEP>
+struct LdsRun {
+    static constexpr bool LINEAR = (STEP * BS) % 16 == 0;
+    static constexpr int DELTA = STEP * BS + (STEP * BS) / 16;
+    int base, i0, b;
+    __device__ __forceinline__ LdsRun(int i0_, int b_) : base(lds_at<BS>(i0_, b_)), i0(i0_), b(b_) {}
+    __device__ __forceinline__ int at(int q) const {
+        if constexpr (LINEAR) return base + q * DELTA;
+        else return lds_at<BS>(i0 + q * STEP, b);
+    }
+};
+
 // One radix-R Stockham stage over B interleaved sequences of length L held in
 // LDS at s[lds_at<BS>(i, b)].  NS = product of the radices of earlier stages.
 // Butterfly beta (b = beta % B, j = beta / B) reads x[j + q L/R], multiplies
@@ -292,8 +309,11 @@ struct StockhamStage {
             if (active(beta)) {
                 const int b = beta % B;
                 const int j = beta / B;
+                {
+                    const LdsRun<BS, LR> run(j, b);
 #pragma unroll
-                for (int q = 0; q < R; ++q) x[r][q] = s[lds_at<BS>(j + q * LR, b)];
+                    for (int q = 0; q < R; ++q) x[r][q] = s[run.at(q)];
+                }
                 if constexpr (NS > 1) {
                     const int k = j % NS;
                     float2 w[R];
@@ -349,11 +369,27 @@ struct StockhamStage {
     }
 
     // in-place stage: returns synchronised
+    // the outputs back into LDS (output q of a butterfly at base + q NS)
+    static __device__ __forceinline__ void emit_lds(float2* s, int tid, const float2 (&x)[NB][R]) {
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int beta = tid + r * NT;
+            if (active(beta)) {
+                const int b = beta % B;
+                const int j = beta / B;
+                const int k = j % NS;
+                const LdsRun<BS, NS> run((j - k) * R + k, b);
+#pragma unroll
+                for (int q = 0; q < R; ++q) s[run.at(q)] = x[r][q];
+            }
+        }
+    }
+
     static __device__ __forceinline__ void run(float2* s, const TwTable<L>& T, int tid) {
         float2 x[NB][R];
         compute(s, T, tid, x);
         __syncthreads();
-        emit(tid, x, [&](int i, int b, float2 v) { s[lds_at<BS>(i, b)] = v; });
+        emit_lds(s, tid, x);
         __syncthreads();
     }
 };
@@ -406,7 +442,7 @@ __device__ __forceinline__ void lds_fft_from(float2* s, const TwTable<L>& T, int
     using St = StockhamStage<L, B, BS, R, 1, NT>;
     float2 x[St::NB][R];
     St::compute_from(src, tid, x);
-    St::emit(tid, x, [&](int i, int b, float2 v) { s[lds_at<BS>(i, b)] = v; });
+    St::emit_lds(s, tid, x);
     __syncthreads();
     LdsFftStages<L, B, BS, R, NT>::run(s, T, tid);
 }

@@ -151,11 +151,11 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     // column FFT; its last stage writes straight from registers to HBM with
     // the inter-pass twiddle exp(2 pi i u2 k1 / n) (consecutive lanes hold
     // consecutive k1 of a column, so the stores coalesce)
-    float2* out = grids + G.grid + (int64_t)gi * n;
+    const __amdgpu_buffer_rsrc_t out = grid_rsrc(grids + G.grid + (int64_t)gi * n, 8u * n);
     lds_fft_to<N1, C1, BS, NT1>(tile, T, tid, [&](int k1, int jj, float2 v) {
         const float2 eh = ehi[jj * NH + (k1 >> 6)];
         const float2 w = (k1 & 63) ? cmul(eh, elo[jj * 64 + (k1 & 63)]) : eh;
-        out[(int64_t)(a + jj) * N1 + k1] = cmul(v, w);
+        grid_store(out, 8u * (uint32_t)((a + jj) * N1 + k1), cmul(v, w));
     });
 }
 
@@ -192,13 +192,14 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
 
     // row FFTs of the tile (rows r0 .. r0+CR-1 of the three grids); the first
     // stage loads its inputs straight from HBM
-    const float2* src = grids + G.grid;
+    const __amdgpu_buffer_rsrc_t src = grid_rsrc(grids + G.grid, 8u * (uint32_t)(NG * n));
     build_twiddles<N2, NT>(T, tid);
     lds_fft_from<N2, B, BS, NT>(tile, T, tid, [&](int u2, int bb) -> float2 {
+        // rows past n1 (a partial last block) read the next column's first
+        // rows, or zero past the region; their transforms are never emitted
         const int gi = bb / CR;
         const int r = bb - gi * CR;
-        return (r0 + r < n1) ? src[(int64_t)gi * n + (int64_t)u2 * n1 + r0 + r]
-                             : make_float2(0.f, 0.f);
+        return grid_load(src, 8u * (uint32_t)(gi * n + u2 * n1 + r0 + r));
     });
 
     const int K = G.nf;
@@ -208,10 +209,16 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const CurveState st = state[ci];
     const float yy32 = (float)st.yy, ybar32 = (float)st.ybar;
     const float rc32 = (float)pp.reg_c, rs32 = (float)pp.reg_s;
-    for (int i = tid; i < N2 * CR; i += NT) {
-        const int r = i % CR;
-        const int k2 = i / CR;
-        if (r0 + r >= n1) continue;
+    // output slots i = tid + q NT: the row r = i % CR is the thread's own
+    // (NT is a multiple of CR), k2 = i / CR steps by NT / CR, and the three
+    // grids' LDS places step by a fixed amount
+    static_assert(NT % CR == 0, "a thread's output slots share one row");
+    constexpr int KSTEP = NT / CR;
+    const int r = tid % CR;
+    const LdsRun<BS, KSTEP> p0(tid / CR, r), p1(tid / CR, CR + r), p2(tid / CR, 2 * CR + r);
+    if (r0 + r >= n1) return;   // a row past n1 (no barrier follows)
+    for (int q = 0; q * KSTEP + tid / CR < N2; ++q) {
+        const int k2 = tid / CR + q * KSTEP;
         const int l = r0 + r + n1 * k2;
         int lp;
         if (l < K - kc) lp = l;
@@ -231,15 +238,15 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
         }
         const int k = lp + kc;
         if (OUT == 2) {
-            const float2 V = cscale(tile[lds_at<BS>(k2, r)], dc);
+            const float2 V = cscale(tile[p0.at(q)], dc);
             // restore the absolute time origin: exp(2 pi i f_k tref)
             const double fk = G.f0 + (double)k * G.df;
             const float2 ph = expi_rev(fk * st.tref);
             ((float2*)out)[G.out + k] = cmul(V, ph);
         } else {
-            const float2 syw = tile[lds_at<BS>(k2, r)];
-            const float2 w1 = tile[lds_at<BS>(k2, CR + r)];
-            const float2 w2 = tile[lds_at<BS>(k2, 2 * CR + r)];
+            const float2 syw = tile[p0.at(q)];
+            const float2 w1 = tile[p1.at(q)];
+            const float2 w2 = tile[p2.at(q)];
             if (OUT == 0) {
                 ((float*)out)[G.out + k] =
                     gls_power_f32(dc * w1.x, dc * w1.y, dc * w2.x, dc * w2.y, dc * syw.x,
