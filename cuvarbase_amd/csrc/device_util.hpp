@@ -108,13 +108,18 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const void* base, ui
 }
 typedef float gls_v2f __attribute__((ext_vector_type(2)));
 typedef unsigned int gls_v2u __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 grid_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const gls_v2f v = __builtin_bit_cast(gls_v2f, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+// byte offset voff (per lane) + soff (wave-uniform: a scalar register, no
+// vector arithmetic); callers keep voff + soff inside the region
+__device__ __forceinline__ float2 grid_load(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                           uint32_t soff = 0) {
+    const gls_v2f v =
+        __builtin_bit_cast(gls_v2f, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
     return make_float2(v.x, v.y);
 }
-__device__ __forceinline__ void grid_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v) {
+__device__ __forceinline__ void grid_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                           float2 v) {
     const gls_v2f x = {v.x, v.y};
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, voff, soff, 0);
 }
 
 // GLS power from the six sums, closed form b^T A^-1 b / den in FP64

@@ -331,7 +331,8 @@ struct StockhamStage {
         }
     }
 
-    // first stage reading its inputs from src(i, b) instead of LDS
+    // first stage reading its inputs from src(j, q, LR, b) -- element j + q LR of
+    // sequence b -- instead of LDS
     template <class S>
     static __device__ __forceinline__ void compute_from(S&& src, int tid,
                                                         float2 (&x)[NB][R]) {
@@ -343,12 +344,30 @@ struct StockhamStage {
                 const int b = beta % B;
                 const int j = beta / B;
 #pragma unroll
-                for (int q = 0; q < R; ++q) x[r][q] = src(j + q * LR, b);
+                for (int q = 0; q < R; ++q) x[r][q] = src(j, q, LR, b);
             }
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r)
             if (active(tid + r * NT)) dft_reg<R>(x[r]);
+    }
+
+    // hand every output to f(base, q, NS, b, v): output q of a butterfly has
+    // index base + q NS in sequence b
+    template <class F>
+    static __device__ __forceinline__ void emit_split(int tid, const float2 (&x)[NB][R], F&& f) {
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int beta = tid + r * NT;
+            if (active(beta)) {
+                const int b = beta % B;
+                const int j = beta / B;
+                const int k = j % NS;
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int q = 0; q < R; ++q) f(base, q, NS, b, x[r][q]);
+            }
+        }
     }
 
     // hand every output (index, sequence, value) to f
@@ -403,7 +422,8 @@ struct LdsFftStages {
             LdsFftStages<L, B, BS, NS * R, NT>::run(s, T, tid);
         }
     }
-    // all stages in place except the last, whose outputs go to sink(i, b, v)
+    // all stages in place except the last, whose outputs go to
+    // sink(base, q, NS, b, v): index base + q NS of sequence b
     // straight from registers (no LDS write, no barrier after it)
     template <class F>
     static __device__ __forceinline__ void run_to(float2* s, const TwTable<L>& T, int tid,
@@ -416,7 +436,7 @@ struct LdsFftStages {
         } else {
             float2 x[St::NB][R];
             St::compute(s, T, tid, x);
-            St::emit(tid, x, sink);
+            St::emit_split(tid, x, sink);
         }
     }
 };
@@ -430,7 +450,8 @@ __device__ __forceinline__ void lds_fft(float2* s, const TwTable<L>& T, int tid)
     LdsFftStages<L, B, BS, 1, NT>::run(s, T, tid);
 }
 
-// The same transform with the first stage's inputs read from src(i, b)
+// The same transform with the first stage's inputs read from src(j, q, LR, b)
+// (element j + q LR of sequence b, LR = L / first radix)
 // (natural order index i of sequence b) rather than from s: the loads of all
 // of a thread's first-stage butterflies are in flight together.  s need not
 // be initialised; T must be complete at the first barrier.  Returns
@@ -447,7 +468,8 @@ __device__ __forceinline__ void lds_fft_from(float2* s, const TwTable<L>& T, int
     LdsFftStages<L, B, BS, R, NT>::run(s, T, tid);
 }
 
-// The same transform with the last stage's outputs handed to sink(i, b, v)
+// The same transform with the last stage's outputs handed to
+// sink(base, q, NS, b, v) (index base + q NS of sequence b)
 // (natural order index i of sequence b) instead of being stored; s is left
 // holding the second-to-last stage.  No barrier at the end.
 template <int L, int B, int BS, int NT, class F>

@@ -152,10 +152,18 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     // the inter-pass twiddle exp(2 pi i u2 k1 / n) (consecutive lanes hold
     // consecutive k1 of a column, so the stores coalesce)
     const __amdgpu_buffer_rsrc_t out = grid_rsrc(grids + G.grid + (int64_t)gi * n, 8u * n);
-    lds_fft_to<N1, C1, BS, NT1>(tile, T, tid, [&](int k1, int jj, float2 v) {
-        const float2 eh = ehi[jj * NH + (k1 >> 6)];
-        const float2 w = (k1 & 63) ? cmul(eh, elo[jj * 64 + (k1 & 63)]) : eh;
-        grid_store(out, 8u * (uint32_t)((a + jj) * N1 + k1), cmul(v, w));
+    // (elo[0] = 1 exactly, so the product needs no special case; when the
+    // output stride ns is a multiple of 64 a butterfly's outputs share one elo
+    // entry and step through ehi, and the stride goes to the scalar offset)
+    lds_fft_to<N1, C1, BS, NT1>(tile, T, tid, [&](int kb, int q, int ns, int jj, float2 v) {
+        float2 w;
+        if ((ns & 63) == 0) {
+            w = cmul(ehi[jj * NH + (kb >> 6) + q * (ns >> 6)], elo[jj * 64 + (kb & 63)]);
+        } else {
+            const int k1 = kb + q * ns;
+            w = cmul(ehi[jj * NH + (k1 >> 6)], elo[jj * 64 + (k1 & 63)]);
+        }
+        grid_store(out, 8u * (uint32_t)((a + jj) * N1 + kb), 8u * (uint32_t)(q * ns), cmul(v, w));
     });
 }
 
@@ -194,12 +202,12 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     // stage loads its inputs straight from HBM
     const __amdgpu_buffer_rsrc_t src = grid_rsrc(grids + G.grid, 8u * (uint32_t)(NG * n));
     build_twiddles<N2, NT>(T, tid);
-    lds_fft_from<N2, B, BS, NT>(tile, T, tid, [&](int u2, int bb) -> float2 {
-        // rows past n1 (a partial last block) read the next column's first
-        // rows, or zero past the region; their transforms are never emitted
+    lds_fft_from<N2, B, BS, NT>(tile, T, tid, [&](int j, int q, int lr, int bb) -> float2 {
+        // column u2 = j + q lr.  Rows past n1 (a partial last block) read
+        // row n1 - 1 instead; their transforms are never emitted
         const int gi = bb / CR;
-        const int r = bb - gi * CR;
-        return grid_load(src, 8u * (uint32_t)(gi * n + u2 * n1 + r0 + r));
+        const int row = min(r0 + bb - gi * CR, n1 - 1);
+        return grid_load(src, 8u * (uint32_t)(gi * n + (j + q * lr) * n1 + row));
     });
 
     const int K = G.nf;
