@@ -188,18 +188,8 @@ __device__ __forceinline__ void dft_reg(float2 (&x)[R]) {
     }
 }
 
-// exp(2 pi i q / n) for an integer q (any sign) and 0 < n <= 2^24, evaluated
-// from the exactly reduced ratio (|2q'| <= n is exact in FP32).
-__device__ __forceinline__ float2 expi_ratio(int64_t q, int64_t n) {
-    int64_t r = q % n;
-    if (r < 0) r += n;
-    if (2 * r >= n) r -= n;
-    float s, c;
-    sincospif((float)(2 * r) / (float)n, &s, &c);
-    return make_float2(c, s);
-}
-// the same for 0 <= q < 2^31 and 0 < n < 2^30, with a 32-bit reduction (the
-// 64-bit remainder by a run-time n is a long instruction sequence)
+// exp(2 pi i q / n) for 0 <= q < 2^31 and 0 < n <= 2^24, evaluated from the
+// exactly reduced ratio (|2q'| <= n is exact in FP32) with a 32-bit remainder
 __device__ __forceinline__ float2 expi_ratio32(uint32_t q, uint32_t n) {
     const uint32_t r = q % n;
     const int rs = (2 * r >= n) ? (int)r - (int)n : (int)r;
@@ -370,24 +360,6 @@ struct StockhamStage {
         }
     }
 
-    // hand every output (index, sequence, value) to f
-    template <class F>
-    static __device__ __forceinline__ void emit(int tid, const float2 (&x)[NB][R], F&& f) {
-#pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            const int beta = tid + r * NT;
-            if (active(beta)) {
-                const int b = beta % B;
-                const int j = beta / B;
-                const int k = j % NS;
-                const int base = (j - k) * R + k;
-#pragma unroll
-                for (int q = 0; q < R; ++q) f(base + q * NS, b, x[r][q]);
-            }
-        }
-    }
-
-    // in-place stage: returns synchronised
     // the outputs back into LDS (output q of a butterfly at base + q NS)
     static __device__ __forceinline__ void emit_lds(float2* s, int tid, const float2 (&x)[NB][R]) {
 #pragma unroll
@@ -404,6 +376,7 @@ struct StockhamStage {
         }
     }
 
+    // in-place stage: returns synchronised
     static __device__ __forceinline__ void run(float2* s, const TwTable<L>& T, int tid) {
         float2 x[NB][R];
         compute(s, T, tid, x);
