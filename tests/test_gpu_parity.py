@@ -489,3 +489,10 @@ def test_captured_graph_replays_match_eager():
     ref = b2.run()
     assert max(float(np.max(np.abs(x - y))) for x, y in zip(got, ref)) < 1e-6
     g.destroy()
+
+
+def test_entry_point_smoke():
+    # the same call the entry point's smoke() makes, against the FP64 oracle
+    import __graft_entry__ as entry
+
+    entry.smoke()

@@ -105,3 +105,12 @@ def test_synthetic_workloads_are_seeded():
     assert len(d[0][0]) == 1000 and len(f[0]) == 10000
     arr, dd, ff = synthetic.stream_requests(100.0, nreq=20)
     assert np.all(np.diff(arr) > 0) and len(dd) == 20
+
+
+def test_smoke_grid_passes_k0_check():
+    # the entry point's smoke run goes through the drop-in's k0 check
+    import __graft_entry__ as entry
+
+    t, y, dy, freqs = entry.smoke_case()
+    ls.check_k0(freqs)
+    assert len(t) == len(y) == len(dy) and np.all(np.diff(t) >= 0)
