@@ -496,3 +496,50 @@ def test_entry_point_smoke():
     import __graft_entry__ as entry
 
     entry.smoke()
+
+
+# one frequency count per pass-1 column length that the grid chooser reaches
+# (every size of csrc/sizes.hpp except 72, 648, 1080, 1152 and 2160, which a
+# cheaper split of the same grid always beats), at sigma = 2
+_CATALOGUE_NF = [512, 640, 768, 800, 864, 1152, 1280, 1536, 2560, 2592, 4096, 4608, 6400,
+                 6912, 8192, 12288, 12800, 12960, 13824, 20480, 24576, 40960, 41472, 65536,
+                 73728, 98304, 102400, 110592, 131072, 1658880]
+
+
+@pytest.mark.parametrize("window", ["gaussian", "es"])
+def test_catalogue_sizes_against_direct_sums(window):
+    """One batch with a curve for (nearly) every pass-1 column length, so
+    every pass-1 kernel and the pass-2 row lengths 64 and 1024 run, including
+    partial row blocks (odd column lengths); powers against FP64 direct sums
+    at the leading frequencies (the low-frequency fix-up) and 48 random ones."""
+    from cuvarbase_amd import engine, tuning
+    from cuvarbase_amd import _lib
+
+    rng = np.random.default_rng(77)
+    data, freqs = [], []
+    for nf in _CATALOGUE_NF:
+        T = float(rng.uniform(50, 500))
+        t = np.sort(rng.uniform(0, T, 80))
+        y = 0.4 * np.sin(2 * np.pi * t / rng.uniform(0.3, 20)) + rng.normal(0, 0.2, t.size)
+        data.append((t, y, np.full(t.size, 0.2)))
+        freqs.append((1 + np.arange(nf)) / (5 * T))
+    if window == "es":
+        kw = dict(sigma=1.35, m=5, window=_lib.WINDOW_ES,
+                  window_beta=tuning.es_beta(1.35, 5, 1.05))
+        tol, ncols = 2e-5, 12     # (the smaller sigma lands on fewer sizes)
+    else:
+        kw = dict(sigma=2.0, m=8)
+        tol, ncols = 1e-5, 25
+    b = engine.LombBatch(data, freqs, **kw)
+    P = b.run()
+    n1s = {b.plan.geometry(i)[1] for i in range(len(data))}
+    n2s = {b.plan.geometry(i)[2] for i in range(len(data))}
+    print(f"{window}: {len(n1s)} column lengths, row lengths {sorted(n2s)}")
+    assert len(n1s) >= ncols
+    worst = 0.0
+    for (t, y, dy), f, p in zip(data, freqs, P):
+        sel = np.concatenate([np.arange(8), rng.choice(len(f), size=48, replace=False)])
+        ref = og.gls_direct(t, y, dy, f[sel])
+        worst = max(worst, float(np.max(np.abs(p[sel].astype(np.float64) - ref))))
+    print(f"{window}: max|dP| = {worst:.3e}")
+    assert worst < tol
