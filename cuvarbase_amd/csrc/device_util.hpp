@@ -121,6 +121,9 @@ __device__ __forceinline__ void grid_store(__amdgpu_buffer_rsrc_t r, uint32_t vo
     const gls_v2f x = {v.x, v.y};
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, voff, soff, 0);
 }
+__device__ __forceinline__ void buf_store_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, 0u, 0);
+}
 
 // GLS power from the six sums, closed form b^T A^-1 b / den in FP64
 // (algebraically equal to lspow / lspow_flmean / lspow0 of
@@ -164,10 +167,11 @@ __device__ __forceinline__ double gls_power(double C, double S, double C2, doubl
 // FP64 from the FP32 terms (the products are exact there, so their
 // cancellation costs no digits beyond those already in CC, SS, CS), and the
 // quotient in FP32.  Frequencies with f (t_max - t_min) < 3, where CC, SS and
-// CS cancel themselves, are recomputed by the FP64 direct sums.
+// CS cancel themselves, are recomputed by the FP64 direct sums.  den_std is
+// the standard mode's denominator YY + ybar^2, formed once by the caller.
 __device__ __forceinline__ float gls_power_f32(float C, float S, float C2, float S2, float YC,
                                                float YS, float YY, float ybar, int mode,
-                                               float l0, float l1) {
+                                               float l0, float l1, float den_std) {
     float CC, SS, CS, bc, bs, den;
     if (mode == GLS_MODE_FLOATING_MEAN) {
         CC = 0.5f * (1.0f + C2) - C * C + l0;
@@ -182,7 +186,7 @@ __device__ __forceinline__ float gls_power_f32(float C, float S, float C2, float
         CS = 0.5f * S2;
         bc = YC + ybar * C;
         bs = YS + ybar * S;
-        den = YY + ybar * ybar;
+        den = den_std;
     } else if (mode == GLS_MODE_WINDOW) {
         CC = 0.5f * (1.0f + C2);
         SS = 0.5f * (1.0f - C2);
@@ -195,7 +199,8 @@ __device__ __forceinline__ float gls_power_f32(float C, float S, float C2, float
     }
     const double cc = CC, ss = SS, cs = CS, xc = bc, xs = bs;
     const double D = cc * ss - cs * cs;
-    const double N = ss * xc * xc + cc * xs * xs - 2.0 * cs * xc * xs;
+    // N = b^T adj(A) b = SS bc^2 + CC bs^2 - 2 CS bc bs
+    const double N = xc * (ss * xc - cs * xs) + xs * (cc * xs - cs * xc);
     const float P = (float)N * __builtin_amdgcn_rcpf(den * (float)D);
     if (!isfinite(P) || P < 0.0f) return -1.0f;
     return P;

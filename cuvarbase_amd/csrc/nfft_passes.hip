@@ -14,6 +14,8 @@
 // (normalize, cuvarbase/kernels/cunfft.cu:243-245) and forms the GLS power
 // (lspow, cuvarbase/kernels/lomb.cu:126-141, in closed form) without ever
 // writing the transforms.
+#include <type_traits>
+
 #include "device_util.hpp"
 #include "lds_fft.hpp"
 
@@ -22,6 +24,9 @@ namespace gls {
 namespace {
 
 constexpr int NT1 = 256;
+
+template <int V>
+using Ic = std::integral_constant<int, V>;
 
 // ---------------------------------------------------------------------------
 // pass 1: spread + column FFT + inter-pass twiddle
@@ -217,56 +222,99 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const CurveState st = state[ci];
     const float yy32 = (float)st.yy, ybar32 = (float)st.ybar;
     const float rc32 = (float)pp.reg_c, rs32 = (float)pp.reg_s;
+    float* const dst32 = (float*)out + G.out;
     // output slots i = tid + q NT: the row r = i % CR is the thread's own
     // (NT is a multiple of CR), k2 = i / CR steps by NT / CR, and the three
     // grids' LDS places step by a fixed amount
     static_assert(NT % CR == 0, "a thread's output slots share one row");
     constexpr int KSTEP = NT / CR;
+    constexpr int QMAX = (N2 + KSTEP - 1) / KSTEP;
     const int r = tid % CR;
-    const LdsRun<BS, KSTEP> p0(tid / CR, r), p1(tid / CR, CR + r), p2(tid / CR, 2 * CR + r);
+    const int k20 = tid / CR;
+    const LdsRun<BS, KSTEP> p0(k20, r), p1(k20, CR + r), p2(k20, 2 * CR + r);
     if (r0 + r >= n1) return;   // a row past n1 (no barrier follows)
-    for (int q = 0; q * KSTEP + tid / CR < N2; ++q) {
-        const int k2 = tid / CR + q * KSTEP;
-        const int l = r0 + r + n1 * k2;
-        int lp;
-        if (l < K - kc) lp = l;
-        else if (l >= n - kc) lp = l - n;
-        else continue;
-        const float xi = (float)lp * ninv;
-        float dc;
-        if (pp.window == GLS_WINDOW_ES) {
-            // 1/psi_hat(|xi|) by linear interpolation in the plan's table
-            const float u = fabsf(xi) * (2.0f * kEsTab);
-            const int i0 = min((int)u, kEsTab);
-            const float f = u - (float)i0;
-            const float t0 = pp.es_deconv[i0];
-            dc = t0 + f * (pp.es_deconv[i0 + 1] - t0);
-        } else {
-            dc = expf(bpi2 * xi * xi);
-        }
-        const int k = lp + kc;
-        if (OUT == 2) {
-            const float2 V = cscale(tile[p0.at(q)], dc);
-            // restore the absolute time origin: exp(2 pi i f_k tref)
-            const double fk = G.f0 + (double)k * G.df;
-            const float2 ph = expi_rev(fk * st.tref);
-            ((float2*)out)[G.out + k] = cmul(V, ph);
-        } else {
-            const float2 syw = tile[p0.at(q)];
-            const float2 w1 = tile[p1.at(q)];
-            const float2 w2 = tile[p2.at(q)];
-            if (OUT == 0) {
-                ((float*)out)[G.out + k] =
-                    gls_power_f32(dc * w1.x, dc * w1.y, dc * w2.x, dc * w2.y, dc * syw.x,
-                                  dc * syw.y, yy32, ybar32, pp.mode, rc32, rs32);
+    const int l0 = r0 + r + n1 * k20;
+    const int lstep = n1 * KSTEP;
+    // byte-addressed views: the three runs' LDS places (linear in q, so the
+    // slot offsets are immediates), the ES table (kEsTab + 2 entries) and this
+    // curve's K power outputs
+    static_assert(LdsRun<BS, KSTEP>::LINEAR, "pass-2 output runs are linear");
+    constexpr int DQ = LdsRun<BS, KSTEP>::DELTA;
+    const float2* const t0p = tile + p0.at(0);
+    const float2* const t1p = tile + p1.at(0);
+    const float2* const t2p = tile + p2.at(0);
+    const __amdgpu_buffer_rsrc_t es_tab = grid_rsrc(pp.es_deconv, 4u * (kEsTab + 2));
+    const __amdgpu_buffer_rsrc_t pw = grid_rsrc(dst32, 4u * (uint32_t)K);
+    const float den_std = yy32 + ybar32 * ybar32;
+    // the slot loop, specialised on the power mode and the window (both
+    // uniform per launch) and unrolled so that the LDS places are immediates
+    auto outputs = [&](auto mode_c, auto es_c) {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr bool ES = decltype(es_c)::value;
+#pragma unroll 8
+        for (int q = 0; q < QMAX; ++q) {
+            if (k20 + q * KSTEP >= N2) break;
+            const int l = l0 + q * lstep;
+            int lp;
+            if (l < K - kc) lp = l;
+            else if (l >= n - kc) lp = l - n;
+            else continue;
+            const float xi = (float)lp * ninv;
+            float dc;
+            if constexpr (ES) {
+                // 1/psi_hat(|xi|) by linear interpolation in the plan's table
+                const float u = fabsf(xi) * (2.0f * kEsTab);
+                const int i0 = min((int)u, kEsTab);
+                const float f = u - (float)i0;
+                const float2 tt = grid_load(es_tab, 4u * (uint32_t)i0);
+                dc = tt.x + f * (tt.y - tt.x);
             } else {
-                // FP64 power output: the assembly in FP64 as well
-                const double d = (double)dc;
-                ((double*)out)[G.out + k] =
-                    gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x, d * syw.y,
-                              st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+                dc = expf(bpi2 * xi * xi);
+            }
+            const uint32_t k = (uint32_t)(lp + kc);
+            if constexpr (OUT == 2) {
+                const float2 V = cscale(t0p[q * DQ], dc);
+                // restore the absolute time origin: exp(2 pi i f_k tref)
+                const double fk = G.f0 + (double)k * G.df;
+                const float2 ph = expi_rev(fk * st.tref);
+                ((float2*)out + G.out)[k] = cmul(V, ph);
+            } else {
+                const float2 syw = t0p[q * DQ];
+                const float2 w1 = t1p[q * DQ];
+                const float2 w2 = t2p[q * DQ];
+                if constexpr (OUT == 0) {
+                    buf_store_f32(pw, 4u * k,
+                                  gls_power_f32(dc * w1.x, dc * w1.y, dc * w2.x, dc * w2.y,
+                                                dc * syw.x, dc * syw.y, yy32, ybar32, MODE,
+                                                rc32, rs32, den_std));
+                } else {
+                    // FP64 power output: the assembly in FP64 as well
+                    const double d = (double)dc;
+                    ((double*)out + G.out)[k] =
+                        gls_power(d * w1.x, d * w1.y, d * w2.x, d * w2.y, d * syw.x, d * syw.y,
+                                  st.yy, st.ybar, pp.mode, pp.reg_c, pp.reg_s);
+                }
             }
         }
+    };
+    using ES_ = std::true_type;
+    using GA_ = std::false_type;
+    if constexpr (OUT == 0) {
+        const int mode = pp.mode;
+        if (pp.window == GLS_WINDOW_ES) {
+            if (mode == GLS_MODE_FLOATING_MEAN) outputs(Ic<GLS_MODE_FLOATING_MEAN>{}, ES_{});
+            else if (mode == GLS_MODE_STANDARD) outputs(Ic<GLS_MODE_STANDARD>{}, ES_{});
+            else if (mode == GLS_MODE_WINDOW) outputs(Ic<GLS_MODE_WINDOW>{}, ES_{});
+            else outputs(Ic<-1>{}, ES_{});
+        } else {
+            if (mode == GLS_MODE_FLOATING_MEAN) outputs(Ic<GLS_MODE_FLOATING_MEAN>{}, GA_{});
+            else if (mode == GLS_MODE_STANDARD) outputs(Ic<GLS_MODE_STANDARD>{}, GA_{});
+            else if (mode == GLS_MODE_WINDOW) outputs(Ic<GLS_MODE_WINDOW>{}, GA_{});
+            else outputs(Ic<-1>{}, GA_{});
+        }
+    } else {
+        if (pp.window == GLS_WINDOW_ES) outputs(Ic<-1>{}, ES_{});
+        else outputs(Ic<-1>{}, GA_{});
     }
 }
 
