@@ -6,8 +6,9 @@ log-uniform in [50, 5000], baselines U[30, 3650] d with seasonal gaps and
 nightly jitter, linear grids df = 1/(5 T) up to 24 /d (1.07e9 curve x
 frequency evaluations) -- run through the batched GPU path with inputs already
 resident in HBM.  Also measured: the plain one-curve-per-call path of the
-drop-in API on a sample, FP64-oracle parity on a sample, per-stage kernel
-times (HIP events on the launch stream) and the FP64 CPU port on the host.
+drop-in API on a sample, per-stage kernel times (HIP events on the launch
+stream), the FP64 CPU port on the host (cpu_baseline), and the parity of the
+GPU periodograms against that port's own outputs on its first sampled curves.
 
 Prints ONE JSON line.
 """
@@ -30,15 +31,20 @@ TOL = 1e-4
 
 
 def _cpu_worker(args):
-    # oracle FP64 NFFT-GLS of one curve (the reference-style CPU path)
+    # oracle FP64 NFFT-GLS of one curve (the reference-style CPU path); the
+    # periodogram is handed back for the curves kept as the parity sample
     from oracle import nfft as onfft
-    t, y, dy, freqs = args
-    onfft.gls_nfft(t, y, dy, freqs, sigma=2.0, m=8)
-    return len(freqs)
+    t, y, dy, freqs, keep = args
+    P = onfft.gls_nfft(t, y, dy, freqs, sigma=2.0, m=8)
+    return len(freqs), (P if keep else None)
 
 
-def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
-    """Time the FP64 oracle port (numpy NFFT + GLS) on a bounded sample."""
+def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8):
+    """Time the FP64 oracle port (numpy NFFT + GLS) on a bounded sample.
+
+    Returns the baseline record and {curve index: oracle periodogram} for the
+    first ``keep`` curves of the sample, against which the GPU results are
+    compared later (the oracle runs only here)."""
     import multiprocessing as mp
 
     cores = max(1, min(16, os.cpu_count() or 1))
@@ -47,7 +53,7 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
     # calibrate on one curve, then size the sample to ~target_s of CPU work
     i0 = int(order[0])
     t0 = time.perf_counter()
-    _cpu_worker((*data[i0], freqs[i0]))
+    _cpu_worker((*data[i0], freqs[i0], False))
     per_eval = (time.perf_counter() - t0) / len(freqs[i0])
     budget_evals = target_s / max(per_eval, 1e-12)
     sample, ev = [], 0
@@ -59,8 +65,11 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
     ctx = mp.get_context("spawn")
     with ctx.Pool(cores) as pool:
         t0 = time.perf_counter()
-        done = sum(pool.map(_cpu_worker, [(*data[i], freqs[i]) for i in sample], chunksize=1))
+        res = pool.map(_cpu_worker, [(*data[i], freqs[i], j < keep)
+                                     for j, i in enumerate(sample)], chunksize=1)
         wall = time.perf_counter() - t0
+    done = sum(r[0] for r in res)
+    kept = {i: r[1] for i, r in zip(sample, res) if r[1] is not None}
     return {
         "value": done / wall,
         "unit": "evals/s",
@@ -70,7 +79,7 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7):
                   f"NumPy NFFT (sigma=2, m=8) + GLS, one curve per task on a "
                   f"{cores}-process pool",
         "wall_s": wall,
-    }
+    }, kept
 
 
 def profiled_traffic(kernel):
@@ -121,7 +130,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ncurves", type=int, default=5000)
     ap.add_argument("--plain-sample", type=int, default=300)
-    ap.add_argument("--parity-sample", type=int, default=8)
+    ap.add_argument("--parity-sample", type=int, default=8,
+                    help="curves of the CPU-baseline sample whose oracle periodograms "
+                         "are kept for the parity check")
+    ap.add_argument("--shapes", type=int, default=1,
+                    help="time the single-curve and long-curve shapes (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sigma", type=float, default=None,
                     help="NFFT oversampling (default: tuned for |dP| <= 1e-4)")
@@ -158,7 +171,8 @@ def main():
     data, freqs = synthetic.survey(args.ncurves)
     total_evals = int(sum(len(f) for f in freqs))
 
-    cpu = None if args.no_cpu else cpu_baseline(data, freqs)
+    cpu, oracle_P = (None, {}) if args.no_cpu else cpu_baseline(data, freqs,
+                                                                 keep=args.parity_sample)
 
     from cuvarbase_amd import engine, _device
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
@@ -231,19 +245,21 @@ def main():
     achieved = (by / c) / (ms / c / 1e3) / 1e9
     traffic = profiled_traffic(dom)
 
-    # FP64 parity on a sample (independent high-accuracy oracle NFFT)
-    from oracle import nfft as onfft
+    # parity on the curves whose FP64 oracle periodograms the CPU-baseline leg
+    # kept (randomly chosen survey curves)
     flat = batch.download_async(stream=s)
     s.synchronize()
     P = batch.split(flat)
-    rng = np.random.default_rng(11)
-    idx = rng.choice(batch.ncurves, size=min(args.parity_sample, batch.ncurves), replace=False)
     max_dp, same = 0.0, 0
-    for i in idx:
-        t, y, dy = data[i]
-        Po = onfft.gls_nfft(t, y, dy, freqs[i], sigma=4.0, m=12)
+    for i, Po in oracle_P.items():
         max_dp = max(max_dp, float(np.max(np.abs(P[i].astype(np.float64) - Po))))
         same += int(np.argmax(P[i]) == np.argmax(Po))
+    parity = None
+    if oracle_P:
+        parity = {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(oracle_P)}",
+                  "tolerance": TOL,
+                  "oracle": "FP64 NumPy NFFT sigma=2 m=8 + GLS (the CPU-baseline port's "
+                            "own outputs on its first sampled curves)"}
 
     # plain path: the drop-in API, one curve per run() + finish()
     proc = LombScargleAsyncProcess(sigma=sigma, m=m)
@@ -328,6 +344,41 @@ def main():
         del sb, tp, ti
         _device.synchronize()
 
+    # one-curve shapes through the same engine, inputs resident: the single
+    # curve (1,000 points x 10,000 frequencies) and the long curve (200,000
+    # points over 3650 d, ~1.8e6 frequencies to 100 /d); eager executes and
+    # replays of the captured execute
+    shapes = None
+    if args.shapes:
+        shapes = {}
+        for name, (cd, cf) in (("single_1000pts_10000freqs", synthetic.single()),
+                               ("long_200000pts_1.8Mfreqs", synthetic.long())):
+            cb = engine.LombBatch(cd, cf, sigma=sigma, m=m, **win)
+            cb.upload(stream=s)
+            cb.execute(stream=s)
+            s.synchronize()
+            e0.record(s)
+            for _ in range(10):
+                cb.execute(stream=s)
+            e1.record(s)
+            e1.synchronize()
+            cms = e1.time_since(e0) / 10
+            g = cb.capture(s)
+            g.launch(s)
+            s.synchronize()
+            e0.record(s)
+            for _ in range(10):
+                g.launch(s)
+            e1.record(s)
+            e1.synchronize()
+            gms = e1.time_since(e0) / 10
+            g.destroy()
+            shapes[name] = {"points": len(cd[0][0]), "evals": int(cb.total_out),
+                            "ms": cms, "evals_per_s": cb.total_out / (cms / 1e3),
+                            "graph_ms": gms}
+            del cb
+        _device.synchronize()
+
     # pipeline-level roofline: fixed algorithmic bytes B = 16N + 24G + 28nf
     nobs = np.array([len(d[0]) for d in data], dtype=np.float64)
     nfa = np.array([len(f) for f in freqs], dtype=np.float64)
@@ -390,10 +441,10 @@ def main():
         "latency_ms": {"p50": ms_step, "p99": ms_step,
                        "note": "every curve of the batch completes with the step"},
         "serial_launch_order_ms": serial_ms,
-        "parity": {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(idx)}",
-                   "oracle": "FP64 NumPy NFFT sigma=4 m=12"},
+        "parity": parity,
         "serving": serving,
         "short_top5": short_leg,
+        "one_curve_shapes": shapes,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
     }
