@@ -153,24 +153,25 @@ __global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __re
 // baseline (CC SS - CS^2 -> 0): there the FP32 round-off of the NFFT sums,
 // largest at the band edge where the lowest frequencies sit, reaches |dP| ~
 // 1e-4.  These few frequencies are recomputed by direct sums with FP64
-// phases and FP64 accumulation (one block per curve, one frequency per
-// thread, points staged through LDS).
+// phases and FP64 accumulation.
 // Grid (curve, group of 4 frequencies); each wave of a block takes one
 // frequency and its 64 lanes stride over the curve's points (FP64 phases,
 // FP64 sums), then reduce by shuffles.  Groups past the curve's low
 // frequencies exit at once.
 constexpr int kLowPerBlock = NT / kWave;
 
-// WIDE: one frequency per block, all NT lanes over the points (long curves)
+// WIDE: one frequency per 1024-thread block, all its lanes over the points
+// (long curves: 16 waves per frequency hide the FP64 latency)
+constexpr int kLowWideNT = 1024;
 template <int OUT, bool WIDE>
-__global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict__ geom,
-                                                     const int32_t* __restrict__ list,
-                                                     const CurveState* __restrict__ state,
-                                                     PointArrays pa, PlanParams pp,
-                                                     void* __restrict__ out) {
-    __shared__ double red[NT / kWave][6];
+__global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
+    const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
+    const CurveState* __restrict__ state, PointArrays pa, PlanParams pp,
+    void* __restrict__ out) {
+    constexpr int BT = WIDE ? kLowWideNT : NT;     // threads per block
+    __shared__ double red[BT / kWave][6];
     constexpr int FPB = WIDE ? 1 : kLowPerBlock;   // frequencies per block
-    constexpr int LANES = WIDE ? NT : kWave;       // lanes per frequency
+    constexpr int LANES = WIDE ? BT : kWave;       // lanes per frequency
     const int tid = threadIdx.x;
     const int lane = tid % LANES;
     const int k = blockIdx.y * FPB + tid / LANES;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(NT) void lowfreq_kernel(const CurveGeom* __restrict
             for (int q = 0; q < 6; ++q) red[wv][q] = acc[q];
         __syncthreads();
         if (tid == 0)
-            for (int w = 1; w < NT / kWave; ++w)
+            for (int w = 1; w < BT / kWave; ++w)
                 for (int q = 0; q < 6; ++q) acc[q] += red[w][q];
     }
     if (lane == 0) {
@@ -254,11 +255,11 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
     if (nwide > 0) {
         const dim3 grid(nwide, kLowMax);
         if (out_f64)
-            hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(NT), 0, s, d_geom, wide,
-                               state, pa, pp, power);
+            hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
+                               wide, state, pa, pp, power);
         else
-            hipLaunchKernelGGL((lowfreq_kernel<0, true>), grid, dim3(NT), 0, s, d_geom, wide,
-                               state, pa, pp, power);
+            hipLaunchKernelGGL((lowfreq_kernel<0, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
+                               wide, state, pa, pp, power);
     }
     return hipGetLastError();
 }
