@@ -222,7 +222,8 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const CurveState st = state[ci];
     const float yy32 = (float)st.yy, ybar32 = (float)st.ybar;
     const float rc32 = (float)pp.reg_c, rs32 = (float)pp.reg_s;
-    float* const dst32 = (float*)out + G.out;
+    // this curve's FP32 powers (OUT == 0)
+    float* const dst32 = (OUT == 0) ? (float*)out + G.out : nullptr;
     // output slots i = tid + q NT: the row r = i % CR is the thread's own
     // (NT is a multiple of CR), k2 = i / CR steps by NT / CR, and the three
     // grids' LDS places step by a fixed amount
@@ -244,7 +245,7 @@ __global__ __launch_bounds__((pass2_threads<P2, OUT>())) void pass2_kernel(
     const float2* const t1p = tile + p1.at(0);
     const float2* const t2p = tile + p2.at(0);
     const __amdgpu_buffer_rsrc_t es_tab = grid_rsrc(pp.es_deconv, 4u * (kEsTab + 2));
-    const __amdgpu_buffer_rsrc_t pw = grid_rsrc(dst32, 4u * (uint32_t)K);
+    const __amdgpu_buffer_rsrc_t pw = grid_rsrc(dst32, (OUT == 0) ? 4u * (uint32_t)K : 0u);
     const float den_std = yy32 + ybar32 * ybar32;
     // the slot loop, specialised on the power mode and the window (both
     // uniform per launch) and unrolled so that the LDS places are immediates
