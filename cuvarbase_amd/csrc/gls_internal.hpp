@@ -121,11 +121,27 @@ constexpr int pass2_rows(int p2) {
 // launchers (defined in the kernel translation units)
 // curves of more than kWidePrep points are prepared by 1024-thread blocks
 constexpr int kWidePrep = 16384;
+// For the NFFT method their point bucketing is spread over many blocks of
+// kWideChunk points each (a counting pass and a scatter pass), with this
+// per-curve record in the workspace: the raw-input weight constants found by
+// the one-block reductions, the count of counting blocks done, and per node
+// set the bucket counts, turned into bucket cursors by the last counting block.
+constexpr int kWideChunk = 2048;
+constexpr int kMaxBuckets = 4096;   // n2 / c1 <= 2^12
+struct WideAux {
+    double ymean;
+    double s0inv;
+    int32_t done;
+    int32_t pad;
+    int32_t hist[2][kMaxBuckets];
+};
+// aux: one WideAux per wide curve (NFFT method), or nullptr to prepare each
+// wide curve in one block; wide_max_obs: the largest n_obs in the wide list
 hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                        const int32_t* wide, int nwide, CurveState* state,
                        const double* t, const void* a, const void* b,
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
-                       int y_is_f32, hipStream_t s);
+                       int y_is_f32, WideAux* aux, int wide_max_obs, hipStream_t s);
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
 hipError_t launch_pass1(int n1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,

@@ -50,6 +50,10 @@ struct gls_plan {
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
+    // per wide curve bucketing records (NFFT method; see WideAux)
+    size_t off_wide = 0;
+    int nwide_aux = 0;
+    int wide_max_obs = 0;
     size_t ws_bytes = 0;
     int64_t total_out = 0;
     // optional per-stage timing (events recorded between launches)
@@ -204,6 +208,14 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         P->off_dt = L.take(sizeof(double) * total_pts);
     }
     P->off_bkt = L.take(sizeof(int32_t) * bkt_ints);
+    P->nwide_aux = 0;
+    P->wide_max_obs = 0;
+    for (const CurveGeom& G : P->geom)
+        if (G.n_obs > kWidePrep) {
+            ++P->nwide_aux;
+            P->wide_max_obs = std::max(P->wide_max_obs, (int)G.n_obs);
+        }
+    if (P->nwide_aux > 0) P->off_wide = L.take(sizeof(WideAux) * (size_t)P->nwide_aux);
     P->off_grid = L.take(sizeof(float2) * grid_elems);
     P->bcurves = bcurves;
     // lanes: buckets own disjoint grid regions unless chunked, so they may run
@@ -430,9 +442,10 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
     pp.es_deconv = P->estab.empty() ? nullptr : (const float*)(base + P->off_estab);
     GLS_HIP_TRY(mark());
     const int32_t* d_prep = (const int32_t*)(base + P->off_prep);
+    WideAux* aux = P->nwide_aux > 0 ? (WideAux*)(base + P->off_wide) : nullptr;
     GLS_HIP_TRY(launch_prep(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
                             P->ncurves - P->nprep_small, state, t, a, b, pa, bkt, pp, y_is_f32,
-                            s));
+                            aux, P->wide_max_obs, s));
     GLS_HIP_TRY(mark());
     if (P->pp.method == GLS_METHOD_NFFT) {
         // fork the buckets over the caller's and the auxiliary streams (not
@@ -612,7 +625,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         const size_t shift = L.cur;
         for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_v1, &P->off_U2,
                           &P->off_fr2, &P->off_v2, &P->off_w, &P->off_yw, &P->off_dt,
-                          &P->off_bkt, &P->off_grid})
+                          &P->off_bkt, &P->off_wide, &P->off_grid})
             *o += shift;
         L.cur += probe.cur;
     } else {
@@ -681,7 +694,9 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     reserve_meta(P, L, bcurves.size(), 0);
     P->off_state = L.take(sizeof(CurveState) * (size_t)nitems);
     const size_t shift = L.cur;
-    for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_bkt, &P->off_grid}) *o += shift;
+    for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_bkt, &P->off_wide,
+                      &P->off_grid})
+        *o += shift;
     L.cur += probe.cur;
     fill_meta(P, bcurves, tiles);
     rc = finish_plan(P, L);

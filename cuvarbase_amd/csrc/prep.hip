@@ -11,7 +11,9 @@
 //            offset and the band-centring modulation exp(2 pi i Fc (t - tref))
 //            of every point, in FP64 before the cast to FP32, then a counting
 //            sort of the points by pass-1 column block (so every pass-1 tile
-//            reads only the points that touch it);
+//            reads only the points that touch it); for curves of more than
+//            16,384 points this phase runs as a counting kernel and a scatter
+//            kernel over chunks of 2,048 points (global atomics per bucket);
 //            (direct method) the FP32 weights and FP64 time offsets.
 #include "device_util.hpp"
 
@@ -43,6 +45,69 @@ __device__ __forceinline__ PrepVals load_point(const double* t, const void* a, c
         p.wv = 1.0;
     }
     return p;
+}
+
+// grid cell U (first of the 2m+1 it touches) of a point at time offset dt in
+// node set g (x = g df dt); yc = its position in cells
+__device__ __forceinline__ int prep_cell(double dt, int g, const CurveGeom& G, int m,
+                                         double& yc) {
+    const double x = dt * ((double)g * G.df);
+    yc = (x - floor(x)) * (double)G.n;
+    return (int)floor(yc) - m;
+}
+
+// weights of a point as FP64: (w, yw) for lomb, (-, y) for nfft
+template <int KIND>
+__device__ __forceinline__ void prep_weights(const PrepVals& p, double ymean, double ybar,
+                                             double s0inv, double& w, double& yw) {
+    if (KIND == 0) {
+        w = s0inv / (p.wv * p.wv);
+        yw = w * (p.yv - ymean - ybar);
+    } else if (KIND == 1) {
+        w = p.wv;
+        yw = p.yv;
+    } else {
+        w = 0.0;
+        yw = p.yv;
+    }
+}
+
+// writes a point's entries into both node sets (one for nfft) and, for the
+// low-frequency sums, its prepared weights in original order; cursor(g,
+// bucket) returns the point's slot in the bucket order
+template <int KIND, typename Cursor>
+__device__ __forceinline__ void prep_emit(const PointArrays& pa, const CurveGeom& G, int j,
+                                          double dt, double w, double yw, double Fc, int m,
+                                          Cursor&& cursor) {
+    constexpr int NG = (KIND == 2) ? 1 : 2;
+    const bool lowfix = KIND != 2 && pa.dt != nullptr;
+    const bool ok = isfinite(dt);
+    if (lowfix) {
+        pa.w[G.pts + j] = ok ? (float)w : 0.f;
+        pa.yw[G.pts + j] = ok ? (float)yw : 0.f;
+        pa.dt[G.pts + j] = ok ? dt : 0.0;
+    }
+    if (!ok) return;   // non-finite times contribute nothing
+    const int n2 = 1 << G.p2;
+#pragma unroll
+    for (int g = 1; g <= NG; ++g) {
+        double yc;
+        const int U = prep_cell(dt, g, G, m, yc);
+        const float fr = (float)(yc - floor(yc));
+        const float2 e = expi_rev((double)g * (Fc * dt));
+        const int pos = cursor(g - 1, (U & (n2 - 1)) / G.c1);
+        const int64_t o = G.pts + pos;
+        if (g == 1) {
+            pa.U1[o] = U;
+            pa.fr1[o] = fr;
+            pa.v0[o] = make_float2((float)yw * e.x, (float)yw * e.y);
+            if (KIND != 2) pa.v1[o] = make_float2((float)w * e.x, (float)w * e.y);
+        } else {
+            pa.U2[o] = U;
+            pa.fr2[o] = fr;
+            pa.v2[o] = make_float2((float)w * e.x, (float)w * e.y);
+        }
+    }
 }
 
 // In-place exclusive scan of h[0 .. nb) by the whole block; returns the total.
@@ -86,7 +151,9 @@ __device__ int block_exclusive_scan(int* h, int nb, int* wsum) {
 
 // NTP threads per curve; curves taken from list[blockIdx.x] (256 threads for
 // most curves, 1024 for long ones)
-template <int KIND, int NTP>
+// SPLIT: reductions only; the bucketing of the NFFT method is left to the
+// chunked counting and scatter kernels (aux[blockIdx.x] is this curve's record)
+template <int KIND, int NTP, bool SPLIT>
 __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__ geom,
                                                    const int32_t* __restrict__ list,
                                                    CurveState* __restrict__ state,
@@ -94,7 +161,8 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
                                                    const void* __restrict__ a,
                                                    const void* __restrict__ b,
                                                    PointArrays pa, int32_t* __restrict__ bkt,
-                                                   int m, int method, int y_is_f32) {
+                                                   int m, int method, int y_is_f32,
+                                                   WideAux* __restrict__ aux) {
     __shared__ double red[NTP / kWave];
     __shared__ int hist[2][4096];
     __shared__ int wsum[NTP / kWave + 1];
@@ -153,18 +221,8 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
     }
     const double tref = tmin;
 
-    // weights of point j as FP64: (w, yw) for lomb, (y, -) for nfft
     auto weights_of = [&](const PrepVals& p, double& w, double& yw) {
-        if (KIND == 0) {
-            w = s0inv / (p.wv * p.wv);
-            yw = w * (p.yv - ymean - ybar);
-        } else if (KIND == 1) {
-            w = p.wv;
-            yw = p.yv;
-        } else {
-            w = 0.0;
-            yw = p.yv;
-        }
+        prep_weights<KIND>(p, ymean, ybar, s0inv, w, yw);
     };
 
     if (method == GLS_METHOD_DIRECT || method == GLS_METHOD_DIRECT_CUSTOM) {
@@ -180,10 +238,25 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
         return;
     }
 
+    if constexpr (SPLIT) {
+        // hand the weight constants to the scatter pass; clear the counts
+        WideAux& X = aux[blockIdx.x];
+        const int nbx = (1 << G.p2) / G.c1;
+        if (tid == 0) {
+            X.ymean = ymean;
+            X.s0inv = s0inv;
+            X.done = 0;
+        }
+        for (int i = tid; i < nbx; i += NTP) {
+            X.hist[0][i] = 0;
+            X.hist[1][i] = 0;
+        }
+        return;
+    }
+
     // ---- phase B: cells, modulation and bucketing of both node sets --------
     // (x = df (t - tref) for SYW and SW(f), 2x for SW(2f)); one counting pass,
     // one block scan per set, one scatter pass
-    const int n = G.n;
     const int p2 = G.p2;
     const int n2 = 1 << p2;
     const int c1 = G.c1;
@@ -196,9 +269,7 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
     }
     __syncthreads();
     auto cell_of = [&](double dt, int g, double& yc) -> int {
-        const double x = dt * ((double)g * G.df);
-        yc = (x - floor(x)) * (double)n;
-        return (int)floor(yc) - m;
+        return prep_cell(dt, g, G, m, yc);
     };
 #pragma unroll 4
     for (int j = tid; j < N; j += NTP) {
@@ -220,39 +291,97 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
         if (tid == 0) bk[nb] = total;
     }
     __syncthreads();
-    const bool lowfix = KIND != 2 && pa.dt != nullptr;
 #pragma unroll 4
     for (int j = tid; j < N; j += NTP) {
         PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
-        const double dt = p.t - tref;
         double w, yw;
         weights_of(p, w, yw);
-        if (lowfix) {
-            // prepared points for the low-frequency direct sums (original order)
-            pa.w[G.pts + j] = isfinite(dt) ? (float)w : 0.f;
-            pa.yw[G.pts + j] = isfinite(dt) ? (float)yw : 0.f;
-            pa.dt[G.pts + j] = isfinite(dt) ? dt : 0.0;
-        }
+        prep_emit<KIND>(pa, G, j, p.t - tref, w, yw, Fc, m,
+                        [&](int gs, int bucket) { return atomicAdd(&hist[gs][bucket], 1); });
+    }
+}
+
+// Bucketing of the wide curves' points over many blocks: blockIdx.y picks the
+// curve (wide[blockIdx.y], record aux[blockIdx.y]), blockIdx.x its chunk of
+// kWideChunk points.  Counting: global atomics into the record's counts; the
+// last block of the curve to finish (the block count is gridDim.x for every
+// curve) scans them into the bucket starts bkt and the cursors.
+template <int KIND>
+__global__ __launch_bounds__(NT) void wide_count_kernel(const CurveGeom* __restrict__ geom,
+                                                        const int32_t* __restrict__ wide,
+                                                        const CurveState* __restrict__ state,
+                                                        const double* __restrict__ t,
+                                                        WideAux* __restrict__ aux,
+                                                        int32_t* __restrict__ bkt, int m) {
+    __shared__ int h[kMaxBuckets];
+    __shared__ int wsum[NT / kWave + 1];
+    __shared__ int last;
+    constexpr int NG = (KIND == 2) ? 1 : 2;
+    const int tid = threadIdx.x;
+    const CurveGeom G = geom[wide[blockIdx.y]];
+    WideAux& X = aux[blockIdx.y];
+    const double tref = state[wide[blockIdx.y]].tref;
+    const int n2 = 1 << G.p2;
+    const int nb = n2 / G.c1;
+    const int j1 = min(G.n_obs, (int)(blockIdx.x + 1) * kWideChunk);
+    for (int j = (int)blockIdx.x * kWideChunk + tid; j < j1; j += NT) {
+        const double dt = t[G.obs + j] - tref;
         if (!isfinite(dt)) continue;
 #pragma unroll
         for (int g = 1; g <= NG; ++g) {
             double yc;
-            const int U = cell_of(dt, g, yc);
-            const float fr = (float)(yc - floor(yc));
-            const float2 e = expi_rev((double)g * (Fc * dt));
-            const int pos = atomicAdd(&hist[g - 1][(U & (n2 - 1)) / c1], 1);
-            const int64_t o = G.pts + pos;
-            if (g == 1) {
-                pa.U1[o] = U;
-                pa.fr1[o] = fr;
-                pa.v0[o] = make_float2((float)yw * e.x, (float)yw * e.y);
-                if (KIND != 2) pa.v1[o] = make_float2((float)w * e.x, (float)w * e.y);
-            } else {
-                pa.U2[o] = U;
-                pa.fr2[o] = fr;
-                pa.v2[o] = make_float2((float)w * e.x, (float)w * e.y);
-            }
+            const int U = prep_cell(dt, g, G, m, yc);
+            atomicAdd(&X.hist[g - 1][(U & (n2 - 1)) / G.c1], 1);
         }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = (atomicAdd(&X.done, 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (!last) return;   // block-uniform
+    __threadfence();
+#pragma unroll
+    for (int g = 1; g <= NG; ++g) {
+        for (int i = tid; i < nb; i += NT)
+            h[i] = __hip_atomic_load(&X.hist[g - 1][i], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int total = block_exclusive_scan<NT>(h, nb, wsum);
+        int32_t* bk = bkt + G.bkt + (int64_t)(g - 1) * (nb + 1);
+        for (int i = tid; i < nb; i += NT) {
+            bk[i] = h[i];
+            X.hist[g - 1][i] = h[i];
+        }
+        if (tid == 0) bk[nb] = total;
+        __syncthreads();
+    }
+}
+
+// Scatter: every point of the chunk to its slot (a global atomic on its
+// bucket's cursor), with the weights of the one-block reductions
+template <int KIND>
+__global__ __launch_bounds__(NT) void wide_scatter_kernel(const CurveGeom* __restrict__ geom,
+                                                          const int32_t* __restrict__ wide,
+                                                          const CurveState* __restrict__ state,
+                                                          const double* __restrict__ t,
+                                                          const void* __restrict__ a,
+                                                          const void* __restrict__ b,
+                                                          PointArrays pa,
+                                                          WideAux* __restrict__ aux, int m,
+                                                          int y_is_f32) {
+    const int tid = threadIdx.x;
+    const CurveGeom G = geom[wide[blockIdx.y]];
+    WideAux& X = aux[blockIdx.y];
+    const CurveState st = state[wide[blockIdx.y]];
+    const double ymean = X.ymean, s0inv = X.s0inv;
+    const double Fc = G.f0 + (double)G.kc * G.df;
+    const int j1 = min(G.n_obs, (int)(blockIdx.x + 1) * kWideChunk);
+    for (int j = (int)blockIdx.x * kWideChunk + tid; j < j1; j += NT) {
+        PrepVals p = load_point<KIND>(t, a, b, G.obs + j, y_is_f32);
+        double w, yw;
+        prep_weights<KIND>(p, ymean, st.ybar, s0inv, w, yw);
+        prep_emit<KIND>(pa, G, j, p.t - st.tref, w, yw, Fc, m,
+                        [&](int gs, int bucket) { return atomicAdd(&X.hist[gs][bucket], 1); });
     }
 }
 
@@ -261,26 +390,52 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
 hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                        const int32_t* wide, int nwide, CurveState* state, const double* t,
                        const void* a, const void* b, PointArrays pa, int32_t* bkt,
-                       const PlanParams& pp, int y_is_f32, hipStream_t s) {
+                       const PlanParams& pp, int y_is_f32, WideAux* aux, int wide_max_obs,
+                       hipStream_t s) {
     const int kind = (pp.kind == 1) ? 2 : (pp.input_kind == GLS_INPUT_RAW ? 0 : 1);
     const int method = (pp.kind == 1) ? GLS_METHOD_NFFT : pp.method;
     const int yf = (pp.kind == 1) ? y_is_f32 : 0;
-#define GLS_PREP_LAUNCH(K, NTP, LIST, COUNT)                                                  \
-    hipLaunchKernelGGL((prep_kernel<K, NTP>), dim3(COUNT), dim3(NTP), 0, s, d_geom, LIST,    \
-                       state, t, a, b, pa, bkt, pp.m, method, yf)
+#define GLS_PREP_LAUNCH(K, NTP, SPLIT, LIST, COUNT)                                           \
+    hipLaunchKernelGGL((prep_kernel<K, NTP, SPLIT>), dim3(COUNT), dim3(NTP), 0, s, d_geom,   \
+                       LIST, state, t, a, b, pa, bkt, pp.m, method, yf, aux)
     if (nsmall > 0) {
-        if (kind == 0) GLS_PREP_LAUNCH(0, 256, small, nsmall);
-        else if (kind == 1) GLS_PREP_LAUNCH(1, 256, small, nsmall);
-        else GLS_PREP_LAUNCH(2, 256, small, nsmall);
+        if (kind == 0) GLS_PREP_LAUNCH(0, 256, false, small, nsmall);
+        else if (kind == 1) GLS_PREP_LAUNCH(1, 256, false, small, nsmall);
+        else GLS_PREP_LAUNCH(2, 256, false, small, nsmall);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (nwide > 0) {
-        if (kind == 0) GLS_PREP_LAUNCH(0, 1024, wide, nwide);
-        else if (kind == 1) GLS_PREP_LAUNCH(1, 1024, wide, nwide);
-        else GLS_PREP_LAUNCH(2, 1024, wide, nwide);
+    if (nwide <= 0) return hipGetLastError();
+    const bool split = aux != nullptr && method == GLS_METHOD_NFFT && wide_max_obs > 0;
+    if (!split) {
+        if (kind == 0) GLS_PREP_LAUNCH(0, 1024, false, wide, nwide);
+        else if (kind == 1) GLS_PREP_LAUNCH(1, 1024, false, wide, nwide);
+        else GLS_PREP_LAUNCH(2, 1024, false, wide, nwide);
+        return hipGetLastError();
     }
+    if (kind == 0) GLS_PREP_LAUNCH(0, 1024, true, wide, nwide);
+    else if (kind == 1) GLS_PREP_LAUNCH(1, 1024, true, wide, nwide);
+    else GLS_PREP_LAUNCH(2, 1024, true, wide, nwide);
 #undef GLS_PREP_LAUNCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid((unsigned)((wide_max_obs + kWideChunk - 1) / kWideChunk), (unsigned)nwide);
+    if (kind == 0) {
+        hipLaunchKernelGGL((wide_count_kernel<0>), grid, dim3(NT), 0, s, d_geom, wide, state, t,
+                           aux, bkt, pp.m);
+        hipLaunchKernelGGL((wide_scatter_kernel<0>), grid, dim3(NT), 0, s, d_geom, wide, state,
+                           t, a, b, pa, aux, pp.m, yf);
+    } else if (kind == 1) {
+        hipLaunchKernelGGL((wide_count_kernel<1>), grid, dim3(NT), 0, s, d_geom, wide, state, t,
+                           aux, bkt, pp.m);
+        hipLaunchKernelGGL((wide_scatter_kernel<1>), grid, dim3(NT), 0, s, d_geom, wide, state,
+                           t, a, b, pa, aux, pp.m, yf);
+    } else {
+        hipLaunchKernelGGL((wide_count_kernel<2>), grid, dim3(NT), 0, s, d_geom, wide, state, t,
+                           aux, bkt, pp.m);
+        hipLaunchKernelGGL((wide_scatter_kernel<2>), grid, dim3(NT), 0, s, d_geom, wide, state,
+                           t, a, b, pa, aux, pp.m, yf);
+    }
     return hipGetLastError();
 }
 

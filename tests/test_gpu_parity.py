@@ -543,3 +543,53 @@ def test_catalogue_sizes_against_direct_sums(window):
         worst = max(worst, float(np.max(np.abs(p[sel].astype(np.float64) - ref))))
     print(f"{window}: max|dP| = {worst:.3e}")
     assert worst < tol
+
+
+def test_wide_curves_chunked_bucketing():
+    """Curves of more than 16,384 points, whose bucketing runs over chunks of
+    2,048 points: raw input (engine), weighted input (drop-in) and the plain
+    NFFT, in a batch mixed with short curves, against FP64 direct sums / DFT."""
+    from cuvarbase_amd import engine
+    from cuvarbase_amd.cunfft import NFFTAsyncProcess
+
+    rng = np.random.default_rng(5)
+    data, freqs = [], []
+    for n, T, P in ((300, 30.0, 1.7), (20_000, 100.0, 0.61), (70_001, 500.0, 3.3),
+                    (16_385, 60.0, 0.27), (500, 40.0, 5.0)):
+        t = np.sort(rng.uniform(0, T, n))
+        y = 1.0 + 0.5 * np.sin(2 * np.pi * t / P) + rng.normal(0, 0.4, n)
+        dy = rng.uniform(0.2, 0.6, n)
+        data.append((t, y, dy))
+        df = 1.0 / (5 * T)
+        freqs.append(df * (3 + np.arange(int(6.0 / df))))
+    out = engine.LombBatch(data, freqs).run()
+    drop = _run(data, freqs)
+    for i, ((t, y, dy), f) in enumerate(zip(data, freqs)):
+        kbest = int(np.argmax(out[i]))
+        sel = np.unique(np.concatenate([np.arange(max(kbest - 10, 0), kbest + 11),
+                                        rng.choice(len(f), 60, replace=False)]))
+        Pd = og.gls_direct(t, y, dy, f[sel])
+        e_eng = np.max(np.abs(out[i][sel] - Pd))
+        e_drop = np.max(np.abs(drop[i][sel] - Pd))
+        print(f"curve {i} n={len(t)} nf={len(f)}: engine {e_eng:.2e}, drop-in {e_drop:.2e}")
+        assert e_eng < TOL and e_drop < TOL
+        assert int(np.argmax(Pd)) == int(np.argmax(out[i][sel]))
+    # a wide curve alone equals the same curve in the mixed batch
+    alone = engine.LombBatch([data[2]], [freqs[2]]).run()[0]
+    assert np.max(np.abs(alone - out[2])) < 1e-5
+
+    # plain NFFT of a wide curve against the direct DFT (scaled times, as in
+    # the NFFT test on the reference's data above)
+    tn = np.sort(rng.uniform(0, 1, 30_000))
+    yn = np.cos(2 * np.pi * 40.0 * tn) + 0.1 * rng.normal(size=tn.size)
+    nf = 1000
+    df = 1.0 / (tn.max() - tn.min())
+    k0 = int(-0.5 * nf * df / df)
+    tsc = (tn - tn.min()) / (tn.max() - tn.min()) - 0.5
+    proc = NFFTAsyncProcess(sigma=5, m=8)
+    g = np.asarray(proc.run([(tsc, yn, nf)], minimum_frequency=k0, samples_per_peak=1)[0])
+    proc.finish()
+    d = on.direct_dft(tsc, yn, k0 + np.arange(nf, dtype=float))
+    err = np.max(np.abs(g - d))
+    print(f"wide nfft: max abs err {err:.3e} (sum|y| = {np.sum(np.abs(yn)):.1f})")
+    assert err < 1e-4 * np.sum(np.abs(yn))
