@@ -13,7 +13,8 @@
 //            sort of the points by pass-1 column block (so every pass-1 tile
 //            reads only the points that touch it); for curves of more than
 //            16,384 points this phase runs as a counting kernel and a scatter
-//            kernel over chunks of 2,048 points (global atomics per bucket);
+//            kernel over chunks of 2,048 points (LDS counts per chunk, one
+//            global atomic per bucket and chunk);
 //            (direct method) the FP32 weights and FP64 time offsets.
 #include "device_util.hpp"
 
@@ -303,26 +304,26 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
 
 // Bucketing of the wide curves' points over many blocks: blockIdx.y picks the
 // curve (wide[blockIdx.y], record aux[blockIdx.y]), blockIdx.x its chunk of
-// kWideChunk points.  Counting: global atomics into the record's counts; the
-// last block of the curve to finish (the block count is gridDim.x for every
-// curve) scans them into the bucket starts bkt and the cursors.
+// kWideChunk points.  Each block counts its chunk in LDS and adds the
+// non-zero counts to the record with one global atomic per bucket (per-point
+// global atomics on a few hundred counters serialise on their cache lines).
+// Counting: the last block of the curve to finish (the block count is
+// gridDim.x for every curve) scans the totals into the bucket starts bkt and
+// the cursors.  Scatter: each block reserves its slots per bucket with one
+// atomic on the cursor, then ranks its points in LDS.
 template <int KIND>
-__global__ __launch_bounds__(NT) void wide_count_kernel(const CurveGeom* __restrict__ geom,
-                                                        const int32_t* __restrict__ wide,
-                                                        const CurveState* __restrict__ state,
-                                                        const double* __restrict__ t,
-                                                        WideAux* __restrict__ aux,
-                                                        int32_t* __restrict__ bkt, int m) {
-    __shared__ int h[kMaxBuckets];
-    __shared__ int wsum[NT / kWave + 1];
-    __shared__ int last;
+__device__ __forceinline__ void wide_chunk_counts(int (&h)[2][kMaxBuckets], const CurveGeom& G,
+                                                  const double* __restrict__ t, double tref,
+                                                  int m) {
     constexpr int NG = (KIND == 2) ? 1 : 2;
     const int tid = threadIdx.x;
-    const CurveGeom G = geom[wide[blockIdx.y]];
-    WideAux& X = aux[blockIdx.y];
-    const double tref = state[wide[blockIdx.y]].tref;
     const int n2 = 1 << G.p2;
     const int nb = n2 / G.c1;
+    for (int i = tid; i < nb; i += NT) {
+        h[0][i] = 0;
+        h[1][i] = 0;
+    }
+    __syncthreads();
     const int j1 = min(G.n_obs, (int)(blockIdx.x + 1) * kWideChunk);
     for (int j = (int)blockIdx.x * kWideChunk + tid; j < j1; j += NT) {
         const double dt = t[G.obs + j] - tref;
@@ -331,9 +332,32 @@ __global__ __launch_bounds__(NT) void wide_count_kernel(const CurveGeom* __restr
         for (int g = 1; g <= NG; ++g) {
             double yc;
             const int U = prep_cell(dt, g, G, m, yc);
-            atomicAdd(&X.hist[g - 1][(U & (n2 - 1)) / G.c1], 1);
+            atomicAdd(&h[g - 1][(U & (n2 - 1)) / G.c1], 1);
         }
     }
+    __syncthreads();
+}
+
+template <int KIND>
+__global__ __launch_bounds__(NT) void wide_count_kernel(const CurveGeom* __restrict__ geom,
+                                                        const int32_t* __restrict__ wide,
+                                                        const CurveState* __restrict__ state,
+                                                        const double* __restrict__ t,
+                                                        WideAux* __restrict__ aux,
+                                                        int32_t* __restrict__ bkt, int m) {
+    __shared__ int h[2][kMaxBuckets];
+    __shared__ int wsum[NT / kWave + 1];
+    __shared__ int last;
+    constexpr int NG = (KIND == 2) ? 1 : 2;
+    const int tid = threadIdx.x;
+    const CurveGeom G = geom[wide[blockIdx.y]];
+    WideAux& X = aux[blockIdx.y];
+    const int nb = (1 << G.p2) / G.c1;
+    wide_chunk_counts<KIND>(h, G, t, state[wide[blockIdx.y]].tref, m);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        for (int i = tid; i < nb; i += NT)
+            if (h[g][i] != 0) atomicAdd(&X.hist[g][i], h[g][i]);
     __threadfence();
     __syncthreads();
     if (tid == 0) last = (atomicAdd(&X.done, 1) == (int)gridDim.x - 1);
@@ -341,24 +365,22 @@ __global__ __launch_bounds__(NT) void wide_count_kernel(const CurveGeom* __restr
     if (!last) return;   // block-uniform
     __threadfence();
 #pragma unroll
-    for (int g = 1; g <= NG; ++g) {
+    for (int g = 0; g < NG; ++g) {
         for (int i = tid; i < nb; i += NT)
-            h[i] = __hip_atomic_load(&X.hist[g - 1][i], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
+            h[g][i] = __hip_atomic_load(&X.hist[g][i], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const int total = block_exclusive_scan<NT>(h, nb, wsum);
-        int32_t* bk = bkt + G.bkt + (int64_t)(g - 1) * (nb + 1);
+        const int total = block_exclusive_scan<NT>(h[g], nb, wsum);
+        int32_t* bk = bkt + G.bkt + (int64_t)g * (nb + 1);
         for (int i = tid; i < nb; i += NT) {
-            bk[i] = h[i];
-            X.hist[g - 1][i] = h[i];
+            bk[i] = h[g][i];
+            X.hist[g][i] = h[g][i];
         }
         if (tid == 0) bk[nb] = total;
         __syncthreads();
     }
 }
 
-// Scatter: every point of the chunk to its slot (a global atomic on its
-// bucket's cursor), with the weights of the one-block reductions
 template <int KIND>
 __global__ __launch_bounds__(NT) void wide_scatter_kernel(const CurveGeom* __restrict__ geom,
                                                           const int32_t* __restrict__ wide,
@@ -369,10 +391,20 @@ __global__ __launch_bounds__(NT) void wide_scatter_kernel(const CurveGeom* __res
                                                           PointArrays pa,
                                                           WideAux* __restrict__ aux, int m,
                                                           int y_is_f32) {
+    __shared__ int h[2][kMaxBuckets];
+    constexpr int NG = (KIND == 2) ? 1 : 2;
     const int tid = threadIdx.x;
     const CurveGeom G = geom[wide[blockIdx.y]];
     WideAux& X = aux[blockIdx.y];
     const CurveState st = state[wide[blockIdx.y]];
+    const int nb = (1 << G.p2) / G.c1;
+    wide_chunk_counts<KIND>(h, G, t, st.tref, m);
+    // this block's first slot in each bucket
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        for (int i = tid; i < nb; i += NT)
+            if (h[g][i] != 0) h[g][i] = atomicAdd(&X.hist[g][i], h[g][i]);
+    __syncthreads();
     const double ymean = X.ymean, s0inv = X.s0inv;
     const double Fc = G.f0 + (double)G.kc * G.df;
     const int j1 = min(G.n_obs, (int)(blockIdx.x + 1) * kWideChunk);
@@ -381,7 +413,7 @@ __global__ __launch_bounds__(NT) void wide_scatter_kernel(const CurveGeom* __res
         double w, yw;
         prep_weights<KIND>(p, ymean, st.ybar, s0inv, w, yw);
         prep_emit<KIND>(pa, G, j, p.t - st.tref, w, yw, Fc, m,
-                        [&](int gs, int bucket) { return atomicAdd(&X.hist[gs][bucket], 1); });
+                        [&](int gs, int bucket) { return atomicAdd(&h[gs][bucket], 1); });
     }
 }
 
