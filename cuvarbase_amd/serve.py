@@ -21,7 +21,9 @@ LDS spreading.
 from __future__ import annotations
 
 import bisect
+import contextlib
 import ctypes as C
+import gc
 import itertools
 import time
 from collections import deque
@@ -264,6 +266,24 @@ class GLSServer:
         self.completed = []
         self.batches_launched = 0
         self.evals_done = 0
+        self._warm_up()
+
+    def _warm_up(self):
+        """One tiny request through the whole path at start-up: the first plan
+        of a configuration (its deconvolution table) and the process's first
+        kernel launches cost tens of milliseconds, which would otherwise land
+        in the first real request's latency."""
+        t = np.linspace(0.0, 10.0, 64)
+        r = self.submit(t, np.sin(t), np.ones_like(t), 0.1 + 0.05 * np.arange(64))
+        try:
+            self.drain()
+        except MemoryError:
+            # a budget too small for even this request: start cold
+            self.queue.clear()
+        r.release()
+        self.completed.clear()
+        self.batches_launched = 0
+        self.evals_done = 0
 
     # -- client side ---------------------------------------------------------
     def submit(self, t, y, dy, freqs, arrival=None, custom_freqs=False, k=0, mode=None,
@@ -459,28 +479,44 @@ class GLSServer:
             if self.poll() == 0:
                 time.sleep(sleep)
 
+    @staticmethod
+    @contextlib.contextmanager
+    def _gc_frozen():
+        # objects that exist when a serving loop starts (the caller's data,
+        # the server) leave the collector's generations, so its occasional
+        # full collections stay short inside the loop
+        froze = gc.get_freeze_count() == 0
+        if froze:
+            gc.freeze()
+        try:
+            yield
+        finally:
+            if froze:
+                gc.unfreeze()
+
     # -- measurement -------------------------------------------------------------
     def serve_stream(self, arrivals, data, freqs, sleep=5e-5, release=False):
         """Replay a request stream in real time: request i arrives arrivals[i]
         seconds after the start.  Returns a stats dict.  ``release=True``
         hands each result back (zero-copy mode) as soon as it completes, as a
         consumer that has taken it would."""
-        t0 = time.perf_counter()
         reqs = []
         i, n = 0, len(data)
-        while i < n or self.busy:
-            now = time.perf_counter() - t0
-            while i < n and arrivals[i] <= now:
-                t, y, dy = data[i]
-                reqs.append(self.submit(t, y, dy, freqs[i], arrival=t0 + arrivals[i]))
-                i += 1
-            done = self.poll()
-            if release and done:
-                for r in self.completed[-done:]:
-                    r.release()
-            if done == 0 and (i >= n or arrivals[i] > time.perf_counter() - t0):
-                time.sleep(sleep)
-        wall = time.perf_counter() - t0
+        with self._gc_frozen():
+            t0 = time.perf_counter()
+            while i < n or self.busy:
+                now = time.perf_counter() - t0
+                while i < n and arrivals[i] <= now:
+                    t, y, dy = data[i]
+                    reqs.append(self.submit(t, y, dy, freqs[i], arrival=t0 + arrivals[i]))
+                    i += 1
+                done = self.poll()
+                if release and done:
+                    for r in self.completed[-done:]:
+                        r.release()
+                if done == 0 and (i >= n or arrivals[i] > time.perf_counter() - t0):
+                    time.sleep(sleep)
+            wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in reqs])
         evals = sum(r.nf for r in reqs)
         return {
@@ -509,22 +545,23 @@ class GLSServer:
                 written += 1
             self.completed.clear()
 
-        for rec in wire.iter_requests(reader):
-            if rec.freqs is not None:
-                self.submit(rec.t, rec.y, rec.dy, rec.freqs, custom_freqs=True, k=rec.k,
-                            mode=rec.mode, rid=rec.id)
-            else:
-                if rec.nf < 1 or not (rec.df > 0):
-                    raise wire.WireError(f"request {rec.id}: bad linear grid")
-                self.submit(rec.t, rec.y, rec.dy, rec.frequencies(), k=rec.k, mode=rec.mode,
-                            rid=rec.id)
-            self.poll()
+        with self._gc_frozen():
+            for rec in wire.iter_requests(reader):
+                if rec.freqs is not None:
+                    self.submit(rec.t, rec.y, rec.dy, rec.freqs, custom_freqs=True, k=rec.k,
+                                mode=rec.mode, rid=rec.id)
+                else:
+                    if rec.nf < 1 or not (rec.df > 0):
+                        raise wire.WireError(f"request {rec.id}: bad linear grid")
+                    self.submit(rec.t, rec.y, rec.dy, rec.frequencies(), k=rec.k,
+                                mode=rec.mode, rid=rec.id)
+                self.poll()
+                flush()
+            while self.busy:
+                if self.poll() == 0:
+                    time.sleep(sleep)
+                flush()
             flush()
-        while self.busy:
-            if self.poll() == 0:
-                time.sleep(sleep)
-            flush()
-        flush()
         return written
 
 

@@ -223,11 +223,34 @@ def test_batches_respect_point_cap(serve):
     srv = serve.GLSServer(hbm_budget=1 << 22, slots=2, max_batch_evals=10**6,
                           max_batch_points=cap)
     assert all(s.h_in.size >= 3 * cap for s in srv.slots)   # staging sized up front
+    before = set(serve._test_device.plans)                  # the start-up warm-up plan
     st = srv.serve_stream(np.zeros(len(data)), data, freqs)
-    plans = serve._test_device.plans.values()
+    plans = [p for h, p in serve._test_device.plans.items() if h not in before]
     assert len(plans) == st["batches"] > 1
     for p in plans:
         pts = sum(c[1] for c in p["curves"])
         assert pts <= cap or len(p["curves"]) == 1
     for r, d, f in zip(st["requests_obj"], data, freqs):
         np.testing.assert_allclose(r.power, og.gls_direct(*d, f), atol=1e-6)
+
+
+def test_server_starts_clean_after_warm_up(serve):
+    """The start-up warm-up request leaves no trace: no results, no batches
+    counted, every arena byte returned."""
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=2, max_batch_evals=10**4, zero_copy=True,
+                          result_buffer_bytes=4 * 4096)
+    assert srv.completed == [] and not srv.busy
+    assert srv.batches_launched == 0 and srv.evals_done == 0
+    assert srv.arena.used == 0 and srv.host_arena.used == 0
+
+
+def test_tiny_budget_skips_warm_up(serve):
+    """A budget too small for the warm-up request still builds a server; the
+    first real request that does not fit raises as before."""
+    srv = serve.GLSServer(hbm_budget=256, slots=1)
+    assert srv.completed == [] and not srv.busy and srv.arena.used == 0
+    data, freqs = _requests(1, seed=9)
+    t, y, dy = data[0]
+    srv.submit(t, y, dy, freqs[0])
+    with pytest.raises(MemoryError, match="budget"):
+        srv.drain()
