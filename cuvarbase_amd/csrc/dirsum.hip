@@ -221,6 +221,64 @@ __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
     }
 }
 
+// Wide curves with a WideAux record: grid (wide curve, frequency, part);
+// each of the kLowSplit workgroups of a low frequency sums one slice of the
+// points, adds its six FP64 sums into the record, and the last to finish
+// forms the power.  Frequencies past the low run exit at once.
+template <int OUT>
+__global__ __launch_bounds__(kLowWideNT) void lowfreq_split_kernel(
+    const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
+    const CurveState* __restrict__ state, PointArrays pa, PlanParams pp,
+    WideAux* __restrict__ aux, void* __restrict__ out) {
+    __shared__ double red[kLowWideNT / kWave][6];
+    const int tid = threadIdx.x;
+    const int k = blockIdx.y;
+    const int c = list[blockIdx.x];
+    const CurveGeom& G = geom[c];
+    const CurveState& st = state[c];
+    if (k >= min(G.nf, kLowMax)) return;
+    const double f = G.f0 + (double)k * G.df;
+    if (fabs(f) * st.tspan >= kLowCycles) return;   // the test of lowfreq_kernel<OUT, true>
+    const int N = G.n_obs;
+    const int per = (N + kLowSplit - 1) / kLowSplit;
+    const int j0 = (int)blockIdx.z * per;
+    const int j1 = min(N, j0 + per);
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = j0 + tid; j < j1; j += kLowWideNT) {
+        const double x = f * pa.dt[G.pts + j];
+        double sn, cs;
+        sincospi(2.0 * (x - rint(x)), &sn, &cs);
+        const double w = pa.w[G.pts + j], yw = pa.yw[G.pts + j];
+        acc[0] += w * cs;
+        acc[1] += w * sn;
+        acc[2] += w * (cs * cs - sn * sn);
+        acc[3] += w * (2.0 * cs * sn);
+        acc[4] += yw * cs;
+        acc[5] += yw * sn;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
+    const int wv = tid / kWave;
+    if ((tid & (kWave - 1)) == 0)
+        for (int q = 0; q < 6; ++q) red[wv][q] = acc[q];
+    __syncthreads();
+    if (tid != 0) return;
+    for (int w = 1; w < kLowWideNT / kWave; ++w)
+        for (int q = 0; q < 6; ++q) acc[q] += red[w][q];
+    WideAux& X = aux[blockIdx.x];
+    for (int q = 0; q < 6; ++q) atomicAdd(&X.low[k][q], acc[q]);
+    __threadfence();
+    if (atomicAdd(&X.low_done[k], 1) != kLowSplit - 1) return;
+    __threadfence();
+    double sm[6];
+    for (int q = 0; q < 6; ++q)
+        sm[q] = __hip_atomic_load(&X.low[k][q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double P = gls_power(sm[0], sm[1], sm[2], sm[3], sm[4], sm[5], st.yy, st.ybar,
+                               pp.mode, pp.reg_c, pp.reg_s);
+    if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
+    else ((double*)out)[G.out + k] = P;
+}
+
 template <int RF>
 hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
                            const int32_t* tile_curve, const int32_t* tile_first, int ntiles,
@@ -240,7 +298,7 @@ hipError_t dirsum_dispatch(const CurveGeom* d_geom, const CurveState* state,
 hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                           const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
-                          hipStream_t s) {
+                          WideAux* aux, hipStream_t s) {
     if (nsmall > 0) {
         const dim3 grid(nsmall, (kLowMax + kLowPerBlock - 1) / kLowPerBlock);
         if (out_f64)
@@ -252,7 +310,15 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (nwide > 0) {
+    if (nwide > 0 && aux != nullptr) {
+        const dim3 grid(nwide, kLowMax, kLowSplit);
+        if (out_f64)
+            hipLaunchKernelGGL((lowfreq_split_kernel<1>), grid, dim3(kLowWideNT), 0, s, d_geom,
+                               wide, state, pa, pp, aux, power);
+        else
+            hipLaunchKernelGGL((lowfreq_split_kernel<0>), grid, dim3(kLowWideNT), 0, s, d_geom,
+                               wide, state, pa, pp, aux, power);
+    } else if (nwide > 0) {
         const dim3 grid(nwide, kLowMax);
         if (out_f64)
             hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(kLowWideNT), 0, s, d_geom,

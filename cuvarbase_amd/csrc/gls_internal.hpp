@@ -128,12 +128,20 @@ constexpr int kWidePrep = 16384;
 // set the bucket counts, turned into bucket cursors by the last counting block.
 constexpr int kWideChunk = 2048;
 constexpr int kMaxBuckets = 4096;   // n2 / c1 <= 2^12
+constexpr double kLowCycles = 3.0;
+constexpr int kLowMax = 256;
+constexpr int kLowSplit = 8;
+// (the record also holds, per low frequency, the six FP64 sums that the
+// kLowSplit workgroups of the low-frequency kernel add into, and how many of
+// them are done; the reductions kernel clears both at every execute)
 struct WideAux {
     double ymean;
     double s0inv;
     int32_t done;
     int32_t pad;
     int32_t hist[2][kMaxBuckets];
+    double low[kLowMax][6];
+    int32_t low_done[kLowMax];
 };
 // aux: one WideAux per wide curve (NFFT method), or nullptr to prepare each
 // wide curve in one block; wide_max_obs: the largest n_obs in the wide list
@@ -160,13 +168,13 @@ hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const PlanParams& pp, void* power, int out_f64,
                          hipStream_t s);
 // direct FP64 sums for a curve's lowest frequencies, f_k (t_max - t_min) <
-// kLowCycles (at most kLowMax of them), overwriting the NFFT powers there
-constexpr double kLowCycles = 3.0;
-constexpr int kLowMax = 256;
+// kLowCycles (at most kLowMax of them), overwriting the NFFT powers there;
+// with aux (one WideAux per wide curve) a wide curve's frequencies are each
+// summed by kLowSplit workgroups
 hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                           const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
-                          hipStream_t s);
+                          WideAux* aux, hipStream_t s);
 hipError_t launch_dirsum_custom(const CurveGeom* d_geom, const CurveState* state,
                                 const int32_t* tile_curve, const int32_t* tile_first,
                                 int ntiles, PointArrays pa, const PlanParams& pp,

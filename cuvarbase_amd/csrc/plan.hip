@@ -488,7 +488,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         if (P->kind == 0) {
             GLS_HIP_TRY(launch_lowfreq(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
                                        P->ncurves - P->nprep_small, state, pa, pp, out,
-                                       out_f64, s));
+                                       out_f64, aux, s));
             GLS_HIP_TRY(mark());
         }
     } else if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM) {

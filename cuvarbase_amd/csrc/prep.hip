@@ -252,6 +252,8 @@ __global__ __launch_bounds__(NTP) void prep_kernel(const CurveGeom* __restrict__
             X.hist[0][i] = 0;
             X.hist[1][i] = 0;
         }
+        for (int i = tid; i < kLowMax * 6; i += NTP) (&X.low[0][0])[i] = 0.0;
+        for (int i = tid; i < kLowMax; i += NTP) X.low_done[i] = 0;
         return;
     }
 
