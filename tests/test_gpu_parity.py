@@ -564,15 +564,18 @@ def test_wide_curves_chunked_bucketing():
         freqs.append(df * (3 + np.arange(int(6.0 / df))))
     out = engine.LombBatch(data, freqs).run()
     drop = _run(data, freqs)
+    drop64 = _run(data, freqs, use_double=True)   # FP64 powers (low frequencies included)
     for i, ((t, y, dy), f) in enumerate(zip(data, freqs)):
         kbest = int(np.argmax(out[i]))
-        sel = np.unique(np.concatenate([np.arange(max(kbest - 10, 0), kbest + 11),
+        sel = np.unique(np.concatenate([np.arange(0, 20), np.arange(max(kbest - 10, 0), kbest + 11),
                                         rng.choice(len(f), 60, replace=False)]))
         Pd = og.gls_direct(t, y, dy, f[sel])
         e_eng = np.max(np.abs(out[i][sel] - Pd))
         e_drop = np.max(np.abs(drop[i][sel] - Pd))
-        print(f"curve {i} n={len(t)} nf={len(f)}: engine {e_eng:.2e}, drop-in {e_drop:.2e}")
-        assert e_eng < TOL and e_drop < TOL
+        e_d64 = np.max(np.abs(drop64[i][sel] - Pd))
+        print(f"curve {i} n={len(t)} nf={len(f)}: engine {e_eng:.2e}, drop-in {e_drop:.2e}, "
+              f"f64 {e_d64:.2e}")
+        assert e_eng < TOL and e_drop < TOL and e_d64 < TOL
         assert int(np.argmax(Pd)) == int(np.argmax(out[i][sel]))
     # a wide curve alone equals the same curve in the mixed batch
     alone = engine.LombBatch([data[2]], [freqs[2]]).run()[0]
