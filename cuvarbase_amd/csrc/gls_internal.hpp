@@ -28,6 +28,18 @@ int hip_fail(hipError_t e, const char* what);
 bool initialised();
 int require_init();
 
+// Asynchronous host-to-device copy of a plan's descriptors through the
+// library's page-locked staging ring (allocated by gls_init): the source is
+// copied into the ring on the host and the DMA reads the ring, so the call
+// never waits on the runtime's shared pageable staging buffer.  Falls back to
+// a plain hipMemcpyAsync for blocks larger than half the ring.
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Device copy of the ES deconvolution table for (m, beta), uploaded once per
+// process and kept until it ends (plans point at it; it is not part of the
+// per-execute descriptors).
+int es_table_device(int m, float beta, const float** out);
+
 // library-owned streams (default priority) onto which an execute forks the
 // independent size buckets of a plan; created on first use
 constexpr int kAuxStreams = 3;
@@ -95,8 +107,8 @@ struct PlanParams {
     int32_t kind;      // 0 = lomb, 1 = nfft
     int32_t window;    // GLS_WINDOW_*
     float es_beta;     // ES shape parameter
-    // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (device copy
-    // in the workspace; set per execute)
+    // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (the
+    // library's resident device copy for (m, beta), es_table_device)
     const float* es_deconv;
 };
 

@@ -43,10 +43,9 @@ struct gls_plan {
     int rf = 1;
     // host image of the descriptors uploaded into workspace[0 .. meta.size())
     std::vector<char> meta;
-    size_t off_geom = 0, off_bgeom = 0, off_tiles = 0, off_estab = 0, off_prep = 0;
+    size_t off_geom = 0, off_bgeom = 0, off_tiles = 0, off_prep = 0;
     std::vector<int32_t> prep_list;   // curves prepared by 256-thread blocks, then wide ones
     int nprep_small = 0;
-    std::vector<float> estab;      // ES deconvolution table (empty for the Gaussian)
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
@@ -321,6 +320,31 @@ const std::vector<float>& es_deconv_table(int m, float beta) {
     return cache.emplace(std::make_pair(m, beta), std::move(tab)).first->second;
 }
 
+}  // namespace
+
+int es_table_device(int m, float beta, const float** out) {
+    static std::mutex mu;
+    static std::map<std::pair<int, float>, float*> dev;
+    const std::vector<float>& tab = es_deconv_table(m, beta);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = dev.find({m, beta});
+    if (it == dev.end()) {
+        float* p = nullptr;
+        const size_t bytes = sizeof(float) * tab.size();
+        GLS_HIP_TRY(hipMalloc((void**)&p, bytes));
+        hipError_t e = hipMemcpy(p, tab.data(), bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return hip_fail(e, "hipMemcpy (ES table)");
+        }
+        it = dev.emplace(std::make_pair(m, beta), p).first;
+    }
+    *out = it->second;
+    return GLS_OK;
+}
+
+namespace {
+
 // Algorithmic bytes per stage: what each kernel must read and write at
 // minimum for its role in the fused pipeline.
 void build_stage_list(gls_plan* P) {
@@ -369,16 +393,12 @@ void reserve_meta(gls_plan* P, Layout& L, size_t nbcurves, size_t ntiles_ints) {
     P->off_geom = L.take(sizeof(CurveGeom) * P->geom.size());
     P->off_prep = L.take(sizeof(int32_t) * P->prep_list.size());
     P->off_bgeom = L.take(sizeof(CurveGeom) * nbcurves);
-    P->off_estab = L.take(sizeof(float) * P->estab.size());
     P->off_tiles = L.take(sizeof(int32_t) * ntiles_ints);
 }
 
 void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
                const std::vector<int32_t>& tiles) {
     P->meta.assign(P->off_tiles + sizeof(int32_t) * tiles.size(), 0);
-    if (!P->estab.empty())
-        std::memcpy(P->meta.data() + P->off_estab, P->estab.data(),
-                    sizeof(float) * P->estab.size());
     if (!P->prep_list.empty())
         std::memcpy(P->meta.data() + P->off_prep, P->prep_list.data(),
                     sizeof(int32_t) * P->prep_list.size());
@@ -420,11 +440,10 @@ PointArrays point_arrays(const gls_plan* P, void* ws) {
 // The whole launch sequence of a plan on stream s.
 int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* out,
              int out_f64, int y_is_f32, void* ws, hipStream_t s, const double* freqs = nullptr,
-             bool upload_meta = true) {
+             bool upload_meta = true, bool allow_fork = true) {
     char* base = (char*)ws;
     if (upload_meta && !P->meta.empty())
-        GLS_HIP_TRY(hipMemcpyAsync(base, P->meta.data(), P->meta.size(), hipMemcpyHostToDevice,
-                                   s));
+        GLS_HIP_TRY(upload_async(base, P->meta.data(), P->meta.size(), s));
     const CurveGeom* d_geom = (const CurveGeom*)(base + P->off_geom);
     const CurveGeom* d_bgeom = (const CurveGeom*)(base + P->off_bgeom);
     const int32_t* d_tiles = (const int32_t*)(base + P->off_tiles);
@@ -438,8 +457,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         return hipEventRecord(P->events[ev++], s);
     };
     const int ngrids = (P->kind == 1) ? 1 : 3;
-    PlanParams pp = P->pp;
-    pp.es_deconv = P->estab.empty() ? nullptr : (const float*)(base + P->off_estab);
+    const PlanParams& pp = P->pp;
     GLS_HIP_TRY(mark());
     const int32_t* d_prep = (const int32_t*)(base + P->off_prep);
     WideAux* aux = P->nwide_aux > 0 ? (WideAux*)(base + P->off_wide) : nullptr;
@@ -452,7 +470,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
         // when timing, whose events need one serial stream), join at the end
         bool fork = false;
         for (int lane : P->bucket_lane) fork |= lane > 0;
-        fork &= P->fork_buckets && !P->timing;
+        fork &= P->fork_buckets && !P->timing && allow_fork;
         hipStream_t lanes[kAuxStreams + 1] = {s};
         if (fork) {
             GLS_HIP_TRY(aux_streams(lanes + 1));
@@ -504,7 +522,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
 }
 
 int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, void* power,
-                 int out_f64, void* ws, void* stream) {
+                 int out_f64, void* ws, void* stream, bool allow_fork = true) {
     int rc = require_init();
     if (rc) return rc;
     if (!P) return fail(GLS_E_INVALID, "plan is NULL");
@@ -513,7 +531,8 @@ int execute_lomb(gls_plan* P, const double* t, const void* a, const void* b, voi
     if (!t || !a || !b || !power || !ws) return fail(GLS_E_INVALID, "NULL device pointer");
     if (P->pp.method == GLS_METHOD_DIRECT_CUSTOM)
         return fail(GLS_E_INVALID, "custom-frequency plans run with gls_lomb_execute_custom");
-    return run_plan(P, t, a, b, power, out_f64, 0, ws, (hipStream_t)stream);
+    return run_plan(P, t, a, b, power, out_f64, 0, ws, (hipStream_t)stream, nullptr, true,
+                    allow_fork);
 }
 
 int finish_plan(gls_plan* P, Layout& L) {
@@ -574,7 +593,14 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.es_beta = prm.window_beta > 0.f
                         ? prm.window_beta
                         : (float)(M_PI * (2 * prm.m + 1) * (1.0 - 0.5 / prm.sigma));
-    if (P->pp.window == GLS_WINDOW_ES) P->estab = es_deconv_table(prm.m, P->pp.es_beta);
+    P->pp.es_deconv = nullptr;
+    if (P->pp.window == GLS_WINDOW_ES) {
+        rc = es_table_device(prm.m, P->pp.es_beta, &P->pp.es_deconv);
+        if (rc != GLS_OK) {
+            destroy_plan(P);
+            return rc;
+        }
+    }
     P->sigma = prm.sigma;
     P->chunk_bytes = std::max<int64_t>(prm.grid_chunk_bytes, 0);
     P->ncurves = ncurves;
@@ -867,9 +893,8 @@ int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a, const 
     const hipStream_t s = (hipStream_t)stream;
     if (!s) return fail(GLS_E_INVALID, "capture needs a created stream (not the null stream)");
     // descriptors go into the workspace once, outside the graph
-    if (!plan->meta.empty())
-        GLS_HIP_TRY(hipMemcpyAsync(workspace, plan->meta.data(), plan->meta.size(),
-                                   hipMemcpyHostToDevice, s));
+    if (!plan->meta.empty()) GLS_HIP_TRY(upload_async(workspace, plan->meta.data(),
+                                                      plan->meta.size(), s));
     GLS_HIP_TRY(hipStreamSynchronize(s));
     const bool timing = plan->timing, fork = plan->fork_buckets;
     plan->timing = false;

@@ -2,7 +2,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
+#include <vector>
 
 #include "gls_internal.hpp"
 
@@ -14,7 +16,99 @@ std::mutex g_mu;
 bool g_init = false;
 int g_device = -1;
 hipStream_t g_aux[kAuxStreams] = {};
+
+// Page-locked staging ring of the descriptor uploads: a byte ring with one
+// event per in-flight span; a span is reused only after its copy completed.
+constexpr size_t kStageBytes = size_t(16) << 20;
+constexpr int kStageEvents = 256;
+struct StageSpan {
+    size_t off, len;
+    int ev;
+};
+struct StageRing {
+    char* base = nullptr;
+    size_t head = 0;
+    std::deque<StageSpan> inflight;
+    hipEvent_t ev[kStageEvents] = {};
+    std::vector<int> free_ev;
+};
+std::mutex g_stage_mu;
+StageRing g_stage;
+
+// with g_mu held and the device current
+int stage_setup() {
+    if (g_stage.base) return GLS_OK;
+    void* p = nullptr;
+    GLS_HIP_TRY(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
+    std::memset(p, 0, kStageBytes);   // fault the pages in now, not inside a copy
+    for (int i = 0; i < kStageEvents; ++i) {
+        hipError_t e = hipEventCreateWithFlags(&g_stage.ev[i], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipEventDestroy(g_stage.ev[j]);
+            (void)hipHostFree(p);
+            return hip_fail(e, "hipEventCreateWithFlags (staging ring)");
+        }
+    }
+    g_stage.free_ev.clear();
+    for (int i = kStageEvents - 1; i >= 0; --i) g_stage.free_ev.push_back(i);
+    g_stage.inflight.clear();
+    g_stage.head = 0;
+    g_stage.base = (char*)p;
+    return GLS_OK;
+}
+
+void stage_teardown() {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!g_stage.base) return;
+    for (auto& sp : g_stage.inflight) (void)hipEventSynchronize(g_stage.ev[sp.ev]);
+    for (auto& e : g_stage.ev) (void)hipEventDestroy(e);
+    (void)hipHostFree(g_stage.base);
+    g_stage = StageRing{};
+}
 }  // namespace
+
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    StageRing& R = g_stage;
+    const size_t need = (bytes + 255) / 256 * 256;
+    if (!R.base || need > kStageBytes / 2)
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    auto release = [&](std::deque<StageSpan>::iterator it) {
+        R.free_ev.push_back(it->ev);
+        return R.inflight.erase(it);
+    };
+    // spans whose copies have finished, oldest first
+    while (!R.inflight.empty() && hipEventQuery(R.ev[R.inflight.front().ev]) == hipSuccess)
+        release(R.inflight.begin());
+    const size_t off = (R.head + need <= kStageBytes) ? R.head : 0;
+    // wait for older copies still reading [off, off + need), and for a free event
+    for (auto it = R.inflight.begin(); it != R.inflight.end();) {
+        if (it->off < off + need && off < it->off + it->len) {
+            hipError_t e = hipEventSynchronize(R.ev[it->ev]);
+            if (e != hipSuccess) return e;
+            it = release(it);
+        } else {
+            ++it;
+        }
+    }
+    if (R.free_ev.empty()) {
+        hipError_t e = hipEventSynchronize(R.ev[R.inflight.front().ev]);
+        if (e != hipSuccess) return e;
+        release(R.inflight.begin());
+    }
+    std::memcpy(R.base + off, src, bytes);
+    hipError_t e = hipMemcpyAsync(dst, R.base + off, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    const int ev = R.free_ev.back();
+    e = hipEventRecord(R.ev[ev], s);
+    if (e != hipSuccess) return e;
+    R.free_ev.pop_back();
+    R.inflight.push_back({off, need, ev});
+    R.head = off + need;
+    return hipSuccess;
+}
+
 
 hipError_t aux_streams(hipStream_t* out) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -87,6 +181,11 @@ int gls_init(int device) {
     if (device < 0 || device >= n)
         return fail(GLS_E_INVALID, "device index out of range");
     GLS_HIP_TRY(hipSetDevice(device));
+    {
+        std::lock_guard<std::mutex> sl(g_stage_mu);
+        int rc = stage_setup();
+        if (rc) return rc;
+    }
     g_device = device;
     g_init = true;
     return GLS_OK;
@@ -95,6 +194,8 @@ int gls_init(int device) {
 int gls_shutdown(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_init) return GLS_OK;
+    (void)hipDeviceSynchronize();
+    stage_teardown();
     for (auto& a : g_aux) {
         if (a) (void)hipStreamDestroy(a);
         a = nullptr;
