@@ -35,6 +35,8 @@ METHOD_DIRECT_CUSTOM = 2
 WINDOW_GAUSSIAN = 0
 WINDOW_ES = 1
 
+STATIC_SCRATCH_BYTES = 128 << 20   # GLS_STATIC_SCRATCH_BYTES
+
 
 class GLSError(RuntimeError):
     """A libgls call returned a non-zero status."""
@@ -114,6 +116,7 @@ _PROTOS = {
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "gls_lomb_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "gls_lomb_execute_static": [_vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_custom": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp],
     "gls_mh_power": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32, _vp],
     "gls_plan_set_fork": [_vp, C.c_int32],

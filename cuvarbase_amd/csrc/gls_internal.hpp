@@ -35,6 +35,11 @@ int require_init();
 // a plain hipMemcpyAsync for blocks larger than half the ring.
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
 
+// Device address and size of the static scratch array of the current device
+// (gls_lomb_execute_static); GLS_E_NOT_INIT before gls_init.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+int static_scratch(void** ptr);
+
 // Device copy of the ES deconvolution table for (m, beta), uploaded once per
 // process and kept until it ends (plans point at it; it is not part of the
 // per-execute descriptors).

@@ -776,6 +776,25 @@ int gls_lomb_execute(gls_plan* plan, const double* t, const void* a, const void*
     return execute_lomb(plan, t, a, b, power, 0, workspace, stream);
 }
 
+int gls_lomb_execute_static(gls_plan* plan, const double* t, const void* a, const void* b,
+                            float* power, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (plan->ws_bytes > kScratchBytes)
+        return fail(GLS_E_UNSUPPORTED,
+                    "plan needs " + std::to_string(plan->ws_bytes) +
+                        " bytes of workspace; the static scratch of gls_lomb_execute_static "
+                        "holds " + std::to_string(kScratchBytes) +
+                        " (run this plan with gls_lomb_execute and a workspace)");
+    void* ws = nullptr;
+    rc = static_scratch(&ws);
+    if (rc) return rc;
+    // buckets in series on the caller's stream (forking would create the
+    // auxiliary streams and events on first use)
+    return execute_lomb(plan, t, a, b, power, 0, ws, stream, false);
+}
+
 int gls_lomb_execute_custom(gls_plan* plan, const double* t, const void* a, const void* b,
                             const double* freqs, void* power, int32_t power_is_f64,
                             void* workspace, void* stream) {

@@ -8,6 +8,14 @@
 
 #include "gls_internal.hpp"
 
+// Workspace of gls_lomb_execute_static: a file-scope device array, one copy
+// per GPU, held from the loading of this code object to the end of the
+// process.  Host code never uses this name as an address: gls_init looks up
+// the array's device address once per device (hipGetSymbolAddress) and
+// launches take that address.
+static_assert(gls::kScratchBytes == GLS_STATIC_SCRATCH_BYTES, "scratch size and header differ");
+static __device__ __align__(256) char scratch[gls::kScratchBytes];
+
 namespace gls {
 
 namespace {
@@ -16,6 +24,9 @@ std::mutex g_mu;
 bool g_init = false;
 int g_device = -1;
 hipStream_t g_aux[kAuxStreams] = {};
+
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};   // device address of `scratch`, per device
 
 // Page-locked staging ring of the descriptor uploads: a byte ring with one
 // event per in-flight span; a span is reused only after its copy completed.
@@ -109,6 +120,13 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
     return hipSuccess;
 }
 
+int static_scratch(void** ptr) {
+    if (!g_init) return fail(GLS_E_NOT_INIT, "gls_init() has not been called");
+    void* p = (g_device >= 0 && g_device < kMaxDevices) ? g_scratch[g_device] : nullptr;
+    if (!p) return fail(GLS_E_NOT_INIT, "no static scratch address for this device");
+    *ptr = p;
+    return GLS_OK;
+}
 
 hipError_t aux_streams(hipStream_t* out) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -181,6 +199,16 @@ int gls_init(int device) {
     if (device < 0 || device >= n)
         return fail(GLS_E_INVALID, "device index out of range");
     GLS_HIP_TRY(hipSetDevice(device));
+    if (device >= kMaxDevices) return fail(GLS_E_UNSUPPORTED, "device index above 63");
+    // once per device, with the device current: the device address of this
+    // GPU's copy of `scratch`
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(GLS_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch[device] = p;
+    }
     {
         std::lock_guard<std::mutex> sl(g_stage_mu);
         int rc = stage_setup();

@@ -196,6 +196,25 @@ int gls_lomb_execute(gls_plan* plan, const double* t, const void* a,
                      const void* b, float* power, void* workspace,
                      void* stream);
 
+/* gls_lomb_execute without a workspace argument and without any allocation:
+ * no reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a static device array of the library, one
+ * per GPU, of GLS_STATIC_SCRATCH_BYTES (128 MiB), held from the library's
+ * loading to the end of the process; plans whose gls_plan_workspace_bytes is
+ * larger fail with GLS_E_UNSUPPORTED.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (different streams, or threads) race
+ * and must be serialised by the caller; a call may begin once the previous
+ * call's work on the device has been ordered before it (same stream, or an
+ * event wait).  Nothing in the array needs clearing between calls: every
+ * execute initialises what it reads.  Size buckets run in series on `stream`
+ * (gls_plan_set_fork is ignored).  Calls no hipMalloc/hipFree; the
+ * descriptors go up through the library's page-locked staging, allocated by
+ * gls_init. */
+#define GLS_STATIC_SCRATCH_BYTES (128u << 20)
+int gls_lomb_execute_static(gls_plan* plan, const double* t, const void* a,
+                            const void* b, float* power, void* stream);
+
 /* Run the independent size buckets of an NFFT plan concurrently on the
  * library's auxiliary streams (forked from and joined back into the execute
  * stream): a few per cent faster on mixed batches; kernel durations then

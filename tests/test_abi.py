@@ -77,6 +77,14 @@ def test_status_codes_match_header():
         assert int(m.group(1)) == val == getattr(_lib, name)
 
 
+def test_static_scratch_size_matches_header():
+    from cuvarbase_amd import _lib
+
+    m = re.search(r"#define GLS_STATIC_SCRATCH_BYTES \((\d+)u << (\d+)\)", open(HEADER).read())
+    assert m
+    assert int(m.group(1)) << int(m.group(2)) == _lib.STATIC_SCRATCH_BYTES
+
+
 def test_missing_library_fails_loudly(tmp_path):
     from cuvarbase_amd import _lib
 
