@@ -301,9 +301,18 @@ class LombBatch:
         _lib.call("gls_topk", self.plan.handle, self.d_ws.ptr, self.d_power.ptr,
                   dmask.ptr if dmask is not None else None, C.c_int32(k), tp.ptr, ti.ptr,
                   _handle(stream))
-        p = tp.get()[: self.ncurves * k].reshape(self.ncurves, k)
-        i = ti.get()[: self.ncurves * k].reshape(self.ncurves, k)
-        return p, i
+        # read back on the stream of the top-k launch (the null stream does not
+        # wait for a non-blocking stream)
+        n = max(self.ncurves * k, 1)
+        p = tp.get_async(pinned_empty(n, np.float32), stream=stream, count=n)
+        i = ti.get_async(pinned_empty(n, np.int32), stream=stream, count=n)
+        if stream is None:
+            _lib.call("gls_device_synchronize")
+        else:
+            _lib.call("gls_stream_synchronize", _handle(stream))
+        nk = self.ncurves * k
+        return (np.array(p[:nk]).reshape(self.ncurves, k),
+                np.array(i[:nk]).reshape(self.ncurves, k))
 
 
 class NFFTBatch:

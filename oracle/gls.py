@@ -19,9 +19,13 @@ modes, regularised)        model matrix (definition, docs/source/lomb.rst:6-37),
 power_rotated              agreement with the closed form (algebraic identity)
 mh_direct_sums / mhgls     H=1 equals the closed form; H>1 equals lstsq with
                            2H+1 columns
-fap_baluev                 special values only (z=0 => fap=1, monotone in z);
-                           otherwise *parity unpinned* (no reference outputs
-                           exist and the reference cannot run here)
+fap_baluev                 tests/test_oracle_fap.py: the single-frequency
+                           tail equals a simulated Beta(1, (N-3)/2); the whole
+                           FAP within [0.95, 1.35] of the simulated maximum
+                           power over a dense grid (plausible mistakes give
+                           0.17-0.76 or 1.6-2.6); the gamma factor's Stirling
+                           limit; the reference notebook's printed 1.09e-4
+                           (+-30%); z=0 => fap=1, monotone in z
 =========================  ====================================================
 """
 from __future__ import annotations
