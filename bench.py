@@ -123,6 +123,77 @@ def measured_copy_gbs(stream, mib=2048, reps=10):
     return 2.0 * nbytes / (ms / 1e3) / 1e9
 
 
+def _parity(gpu, oracle):
+    """max |dP| and best-index agreement of {curve: powers} against the
+    oracle's periodograms of the same curves."""
+    if not oracle:
+        return None
+    worst, same = 0.0, 0
+    for i, Po in oracle.items():
+        Pg = np.asarray(gpu[i], dtype=np.float64)
+        worst = max(worst, float(np.max(np.abs(Pg - Po))))
+        same += int(np.argmax(Pg) == np.argmax(Po))
+    return {"max_abs_dP": worst, "best_period_match": f"{same}/{len(oracle)}",
+            "tolerance": TOL,
+            "oracle": "FP64 NumPy NFFT sigma=2 m=8 + GLS (the CPU-baseline leg's own outputs "
+                      "on its first sampled curves)"}
+
+
+def end_to_end(batch, stream, reps=3, k=5):
+    """Batch throughput with the inputs starting and the results ending in
+    page-locked host memory: H2D of t, y, dy, the execute, and D2H of every
+    power (or of the k highest peaks per curve).  HIP events on ``stream``."""
+    import ctypes as C
+
+    from cuvarbase_amd import _device, _lib
+
+    h = _device._handle(stream)
+    t, a, b = batch._host
+    hin = [_device.pinned_empty(len(x), x.dtype) for x in (t, a, b)]
+    for dst, src in zip(hin, (t, a, b)):
+        dst[:] = src
+    hout = _device.pinned_empty(max(batch.total_out, 1), np.float32)
+    ncur = batch.ncurves
+    tp = _device.DeviceArray(ncur * k, np.float32)
+    ti = _device.DeviceArray(ncur * k, np.int32)
+    htop = _device.pinned_empty(2 * ncur * k, np.float32)
+    e0, e1 = _device.Event(), _device.Event()
+
+    def upload():
+        for dev, src in zip((batch.d_t, batch.d_a, batch.d_b), hin):
+            _lib.call("gls_memcpy_h2d_async", dev.ptr, src.ctypes.data, C.c_size_t(src.nbytes), h)
+
+    def full():
+        upload()
+        batch.execute(stream=stream)
+        _lib.call("gls_memcpy_d2h_async", hout.ctypes.data, batch.d_power.ptr,
+                  C.c_size_t(4 * batch.total_out), h)
+
+    def peaks():
+        upload()
+        batch.execute(stream=stream)
+        _lib.call("gls_topk", batch.plan.handle, batch.d_ws.ptr, batch.d_power.ptr, None,
+                  C.c_int32(k), tp.ptr, ti.ptr, h)
+        _lib.call("gls_memcpy_d2h_async", htop.ctypes.data, tp.ptr, C.c_size_t(4 * ncur * k), h)
+        _lib.call("gls_memcpy_d2h_async", htop.ctypes.data + 4 * ncur * k, ti.ptr,
+                  C.c_size_t(4 * ncur * k), h)
+
+    out = {}
+    for name, fn in (("all_powers", full), ("top5_peaks", peaks)):
+        fn()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e1.time_since(e0) / reps
+        out[name] = {"ms": ms, "evals_per_s": batch.total_out / (ms / 1e3),
+                     "d2h_bytes": 4 * batch.total_out if name == "all_powers" else 8 * ncur * k}
+    out["h2d_bytes"] = int(sum(x.nbytes for x in hin))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,8 +212,9 @@ def main():
     ap.add_argument("--m", type=int, default=None)
     ap.add_argument("--window", type=int, default=None,
                     help="NFFT window: 0 Gaussian, 1 exponential of semicircle (default: tuned)")
-    ap.add_argument("--stream-requests", type=int, default=1500,
-                    help="requests per arrival rate in the serving leg (0: skip)")
+    ap.add_argument("--stream-requests", type=int, default=5000,
+                    help="requests per arrival rate in the serving leg, at most 10 s of "
+                         "arrivals per rate (0: skip)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--short-curves", type=int, default=100_000,
                     help="curves in the short-curve top-5 leg (0: skip)")
@@ -250,19 +322,18 @@ def main():
     flat = batch.download_async(stream=s)
     s.synchronize()
     P = batch.split(flat)
-    max_dp, same = 0.0, 0
-    for i, Po in oracle_P.items():
-        max_dp = max(max_dp, float(np.max(np.abs(P[i].astype(np.float64) - Po))))
-        same += int(np.argmax(P[i]) == np.argmax(Po))
-    parity = None
-    if oracle_P:
-        parity = {"max_abs_dP": max_dp, "best_period_match": f"{same}/{len(oracle_P)}",
-                  "tolerance": TOL,
-                  "oracle": "FP64 NumPy NFFT sigma=2 m=8 + GLS (the CPU-baseline port's "
-                            "own outputs on its first sampled curves)"}
+    parity = _parity({i: P[i] for i in oracle_P}, oracle_P)
 
-    # plain path: the drop-in API, one curve per run() + finish()
-    proc = LombScargleAsyncProcess(sigma=sigma, m=m)
+    # end to end: the same batch from page-locked host memory to page-locked
+    # host memory (H2D of t, y, dy + execute + D2H of every power, or of the
+    # top-5 peaks per curve), on the execute stream
+    e2e = end_to_end(batch, s, reps=max(1, min(args.steps, 3)))
+
+    # plain path: the drop-in API, one curve per run() + finish(), at the same
+    # NFFT configuration as the batched path
+    plain_kw = dict(sigma=sigma, m=m, nfft_window="es" if cfg["window"] == 1 else "gaussian",
+                    window_beta=cfg["window_beta"])
+    proc = LombScargleAsyncProcess(**plain_kw)
     nplain = min(args.plain_sample, len(data))
     proc.run([data[0]], freqs=[freqs[0]])
     proc.finish()
@@ -273,6 +344,12 @@ def main():
         proc.finish()
         ev += len(freqs[i])
     plain = ev / (time.perf_counter() - t0) if nplain else None
+    plain_P = {}
+    for i in oracle_P:
+        (_, p), = proc.run([data[i]], freqs=[freqs[i]])
+        proc.finish()
+        plain_P[i] = np.array(p)
+    plain_parity = _parity(plain_P, oracle_P)
 
     # served path: continuous batching of host-resident requests arriving as a
     # Poisson stream (PCIe transfers included), and the plain path on the same
@@ -286,11 +363,12 @@ def main():
         arr, sdata, sfreqs = synthetic.stream_requests(1.0, nreq=args.stream_requests)
         srv = GLSServer(hbm_budget=args.hbm_budget_gb * 1e9, slots=3, sigma=sigma, m=m,
                         zero_copy=True, max_batch_points=8_000_000, **win)
-        serving = {"requests_per_rate": args.stream_requests, "hbm_budget_gb": args.hbm_budget_gb,
-                   "rates": {}}
+        serving = {"requests_per_rate": args.stream_requests,
+                   "rule": "per rate: the requested count or 10 s of arrivals, whichever "
+                           "is fewer", "hbm_budget_gb": args.hbm_budget_gb, "rates": {}}
         for rate in (100.0, 1000.0, 10000.0, float("inf")):
-            nreq = args.stream_requests if rate > 500 else min(args.stream_requests,
-                                                             int(rate * 10))
+            nreq = args.stream_requests if math.isinf(rate) else \
+                min(args.stream_requests, int(rate * 10))
             a = np.zeros(nreq) if math.isinf(rate) else arr[:nreq] / rate
             st = srv.serve_stream(a, sdata[:nreq], sfreqs[:nreq], release=True)
             srv.completed.clear()
@@ -305,8 +383,14 @@ def main():
             proc.finish()
             ev += len(sfreqs[i])
         serving["plain_evals_per_s"] = ev / (time.perf_counter() - t0)
+        serving["plain_config"] = dict(plain_kw, path="LombScargleAsyncProcess.run, one request "
+                                       "per call + finish(), PCIe included")
         best = max(v["evals_per_s"] for v in serving["rates"].values())
         serving["served_vs_plain"] = best / serving["plain_evals_per_s"]
+        for rate in serving["rates"].values():
+            rate["vs_plain"] = rate["evals_per_s"] / serving["plain_evals_per_s"]
+        del srv
+        _device.synchronize()
 
     # short-curve leg: 100,000 curves of 20-300 points on one shared grid of
     # 20,000 frequencies, top-5 peaks per curve on the device (inputs resident)
@@ -340,8 +424,24 @@ def main():
         sms = e1.time_since(e0) / 3
         short_leg = {"curves": sb.ncurves, "evals": int(sb.total_out), "ms": sms,
                      "evals_per_s": sb.total_out / (sms / 1e3), "top_k": k,
+                     "path": "engine, inputs resident, one execute + top-5 on the device",
                      "workload": "N~U{20..300}, t~U[0,90] d, 20,000 freqs in [0.01, 40] /d"}
         del sb, tp, ti
+        _device.synchronize()
+        # the same 100,000 requests through the server as one burst: batches
+        # on the shared grid replay captured executes, top-5 per request
+        from cuvarbase_amd.serve import GLSServer
+
+        srv5 = GLSServer(hbm_budget=min(args.hbm_budget_gb, 100.0) * 1e9, slots=3,
+                         sigma=sigma, m=m,
+                         zero_copy=True, max_batch_points=8_000_000, **win)
+        st5 = srv5.serve_stream(np.zeros(len(shd)), shd, shf, release=True, k=k)
+        short_leg["served"] = {key: st5[key] for key in (
+            "requests", "evals", "wall_s", "evals_per_s", "p50_ms", "p99_ms", "batches",
+            "graph_captures", "graph_replays")}
+        short_leg["served"]["path"] = ("GLSServer burst, PCIe included, k=5 peaks per "
+                                       "request, captured executes replayed per shape")
+        del srv5, st5
         _device.synchronize()
 
     # one-curve shapes through the same engine, inputs resident: the single
@@ -379,11 +479,19 @@ def main():
             del cb
         _device.synchronize()
 
-    # pipeline-level roofline: fixed algorithmic bytes B = 16N + 24G + 28nf
+    # pipeline-level traffic at the executed grid sizes: the fused pipeline's
+    # minimum (inputs, one write and one read of the three pass-1 grids per
+    # curve, powers) over the step time
     nobs = np.array([len(d[0]) for d in data], dtype=np.float64)
     nfa = np.array([len(f) for f in freqs], dtype=np.float64)
-    G = np.ceil(2.0 * nfa) + np.ceil(2.0 * (2 * nfa + 1))
-    pipe_bytes = float(np.sum(16 * nobs + 24 * G + 28 * nfa))
+    cells = np.array([3.0 * g[0] for g in geo])
+    pipe_bytes = float(np.sum(24 * nobs + 16 * cells + 4 * nfa))
+
+    latency = None
+    if serving is not None:
+        r = serving["rates"]["10000/s"]
+        latency = {"p50": r["p50_ms"], "p99": r["p99_ms"],
+                   "source": "served requests at 10,000 arrivals/s (serving.rates)"}
 
     out = {
         "metric": "LS evals/s (curve x freq), batched GPU path, survey batch",
@@ -428,8 +536,11 @@ def main():
             "measured_copy_GBs": copy_gbs,
             "frac_of_measured_copy": achieved / copy_gbs,
         },
-        "pipeline_roofline": {
-            "algorithmic_bytes_per_step": pipe_bytes,
+        "pipeline_traffic": {
+            "definition": "24 B per observation (t, y, dy) + 16 B per cell of the three "
+                          "grids at the executed sizes (pass-1 write, pass-2 read) + 4 B "
+                          "per power, over the step time",
+            "bytes_per_step": pipe_bytes,
             "achieved_GBs": pipe_bytes / (ms_step / 1e3) / 1e9,
             "frac": pipe_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
@@ -437,9 +548,11 @@ def main():
         "kernels_ms_serial": {k: v[1] / n_instr for k, v in
                                 sorted(by_kernel.items(), key=lambda kv: -kv[1][1])[:8]},
         "plain_evals_per_s": plain,
-        "batched_vs_plain": (value / plain) if plain else None,
-        "latency_ms": {"p50": ms_step, "p99": ms_step,
-                       "note": "every curve of the batch completes with the step"},
+        "plain_config": plain_kw,
+        "plain_parity": plain_parity,
+        "device_resident_batched_vs_plain": (value / plain) if plain else None,
+        "end_to_end": e2e,
+        "latency_ms": latency,
         "serial_launch_order_ms": serial_ms,
         "parity": parity,
         "serving": serving,

@@ -65,7 +65,7 @@ class gls_params(C.Structure):
                 ("reg_c", C.c_double), ("reg_s", C.c_double),
                 ("input_kind", C.c_int32), ("method", C.c_int32),
                 ("window", C.c_int32), ("window_beta", C.c_float),
-                ("grid_chunk_bytes", C.c_int64)]
+                ("grid_chunk_bytes", C.c_int64), ("point_capacity", C.c_int64)]
 
 
 class gls_mh_curve(C.Structure):
@@ -123,6 +123,8 @@ _PROTOS = {
     "gls_plan_graph_create": [_vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
     "gls_graph_launch": [_vp, _vp],
     "gls_graph_destroy": [_vp],
+    "gls_plan_rebind": [_vp, C.POINTER(gls_curve), C.c_int32],
+    "gls_graph_update": [_vp, _vp, _vp],
     "gls_nfft_plan_create": [C.POINTER(_vp), C.POINTER(gls_nfft_item), C.c_int32,
                              C.c_double, C.c_int32],
     "gls_nfft_execute": [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],

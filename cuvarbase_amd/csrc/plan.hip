@@ -17,13 +17,16 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 
 #include "gls_internal.hpp"
 
 struct gls_plan {
+    uint64_t id = 0;       // unique per plan (graphs remember the plan they came from)
     int kind = 0;  // 0 lomb, 1 nfft
+    int64_t pts_cap = 0;   // points the per-point workspace regions hold
     gls::PlanParams pp{};
     double sigma = 0.0;
     int ncurves = 0;
@@ -70,6 +73,8 @@ namespace gls {
 namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+std::atomic<uint64_t> g_plan_ids{1};
 
 struct Layout {
     size_t cur = 0;
@@ -413,6 +418,17 @@ void fill_meta(gls_plan* P, const std::vector<int32_t>& bcurves,
         std::memcpy(P->meta.data() + P->off_tiles, tiles.data(), sizeof(int32_t) * tiles.size());
 }
 
+// rewrites the geometry regions of the host descriptors (curve order and
+// bucket order) from P->geom; the prep list and tiles are unchanged
+void refresh_meta_geometry(gls_plan* P) {
+    if (!P->geom.empty())
+        std::memcpy(P->meta.data() + P->off_geom, P->geom.data(),
+                    sizeof(CurveGeom) * P->geom.size());
+    for (size_t i = 0; i < P->bcurves.size(); ++i)
+        std::memcpy(P->meta.data() + P->off_bgeom + i * sizeof(CurveGeom),
+                    &P->geom[P->bcurves[i]], sizeof(CurveGeom));
+}
+
 void destroy_plan(gls_plan* P) {
     if (!P) return;
     for (auto e : P->events) (void)hipEventDestroy(e);
@@ -580,7 +596,10 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         return fail(GLS_E_INVALID, "unknown window");
     if (!std::isfinite(prm.window_beta) || prm.window_beta > 200.f)
         return fail(GLS_E_INVALID, "window_beta must be finite and <= 200");
+    if (prm.point_capacity > ((int64_t)1 << 34))
+        return fail(GLS_E_INVALID, "point_capacity above 2^34");
     gls_plan* P = new gls_plan();
+    P->id = g_plan_ids.fetch_add(1);
     P->kind = 0;
     P->pp.m = prm.m;
     P->pp.mode = prm.mode;
@@ -629,6 +648,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         pts += c.n_obs;
         P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
     }
+    P->pts_cap = std::max<int64_t>(pts, prm.point_capacity);
     std::vector<int32_t> bcurves, tiles;
     if (prm.method != GLS_METHOD_NFFT) rc = plan_direct_tiles(P, tiles);
     if (rc != GLS_OK) {
@@ -640,7 +660,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     Layout L;
     if (prm.method == GLS_METHOD_NFFT) {
         Layout probe;
-        rc = plan_nfft_geometry(P, 3, probe, pts, bcurves);
+        rc = plan_nfft_geometry(P, 3, probe, P->pts_cap, bcurves);
         if (rc != GLS_OK) {
             destroy_plan(P);
             return rc;
@@ -657,9 +677,9 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     } else {
         reserve_meta(P, L, 0, tiles.size());
         P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
-        P->off_w = L.take(sizeof(float) * pts);
-        P->off_yw = L.take(sizeof(float) * pts);
-        P->off_dt = L.take(sizeof(double) * pts);
+        P->off_w = L.take(sizeof(float) * P->pts_cap);
+        P->off_yw = L.take(sizeof(float) * P->pts_cap);
+        P->off_dt = L.take(sizeof(double) * P->pts_cap);
     }
     fill_meta(P, bcurves, tiles);
     rc = finish_plan(P, L);
@@ -681,6 +701,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     if (!(sigma >= 1.0 && sigma <= 64.0)) return fail(GLS_E_INVALID, "sigma must be in [1, 64]");
     if (m < 1 || m > 24) return fail(GLS_E_INVALID, "m must be in [1, 24]");
     gls_plan* P = new gls_plan();
+    P->id = g_plan_ids.fetch_add(1);
     P->kind = 1;
     P->pp.m = m;
     P->pp.mode = GLS_MODE_FLOATING_MEAN;
@@ -710,9 +731,10 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
         pts += c.n_obs;
         P->total_out = std::max<int64_t>(P->total_out, c.out_offset + c.nf);
     }
+    P->pts_cap = pts;
     std::vector<int32_t> bcurves, tiles;
     Layout probe, L;
-    rc = plan_nfft_geometry(P, 1, probe, pts, bcurves);
+    rc = plan_nfft_geometry(P, 1, probe, P->pts_cap, bcurves);
     if (rc != GLS_OK) {
         destroy_plan(P);
         return rc;
@@ -896,6 +918,9 @@ int gls_topk(const gls_plan* plan, const void* workspace, const float* power,
 struct gls_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t plan_id = 0;     // the plan whose descriptors the graph's kernels read
+    void* workspace = nullptr;
+    size_t meta_bytes = 0;
 };
 
 int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a, const void* b,
@@ -940,7 +965,73 @@ int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a, const 
         delete g;
         return rc;
     }
+    g->plan_id = plan->id;
+    g->workspace = workspace;
+    g->meta_bytes = plan->meta.size();
     *out = g;
+    return GLS_OK;
+}
+
+int gls_plan_rebind(gls_plan* plan, const gls_curve* curves, int32_t ncurves) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (plan->kind != 0) return fail(GLS_E_INVALID, "plan is not a Lomb-Scargle plan");
+    if (ncurves != plan->ncurves)
+        return fail(GLS_E_INVALID, "rebind needs as many curves as the plan (" +
+                                       std::to_string(plan->ncurves) + ")");
+    if (ncurves > 0 && !curves) return fail(GLS_E_INVALID, "curves is NULL");
+    const bool custom = plan->pp.method == GLS_METHOD_DIRECT_CUSTOM;
+    int64_t pts = 0;
+    int wide_max = 0;
+    for (int i = 0; i < ncurves; ++i) {
+        const gls_curve& c = curves[i];
+        const CurveGeom& G = plan->geom[i];
+        if (check_curve(c.obs_offset, c.n_obs, c.nf, custom ? 0.0 : c.f0, custom ? 1.0 : c.df,
+                        c.out_offset))
+            return fail(GLS_E_INVALID, "invalid curve descriptor " + std::to_string(i));
+        if (c.nf != G.nf || (!custom && (c.f0 != G.f0 || c.df != G.df)))
+            return fail(GLS_E_INVALID, "curve " + std::to_string(i) +
+                                           ": a rebind keeps every curve's frequency grid");
+        if ((c.n_obs > kWidePrep) != (G.n_obs > kWidePrep))
+            return fail(GLS_E_INVALID, "curve " + std::to_string(i) +
+                                           " crosses the 16,384-point preparation limit");
+        if (c.n_obs > kWidePrep) wide_max = std::max(wide_max, (int)c.n_obs);
+        pts += c.n_obs;
+    }
+    if (pts > plan->pts_cap)
+        return fail(GLS_E_INVALID, "rebind of " + std::to_string(pts) +
+                                       " points exceeds the plan's point capacity " +
+                                       std::to_string(plan->pts_cap));
+    if (wide_max > plan->wide_max_obs)
+        return fail(GLS_E_INVALID, "a curve above 16,384 points is longer than the plan's");
+    int64_t off = 0, total_out = 0;
+    for (int i = 0; i < ncurves; ++i) {
+        const gls_curve& c = curves[i];
+        CurveGeom& G = plan->geom[i];
+        G.obs = c.obs_offset;
+        G.n_obs = c.n_obs;
+        G.pts = off;
+        G.out = c.out_offset;
+        G.ybar_in = c.ybar;
+        G.yy_in = c.yy;
+        off += c.n_obs;
+        total_out = std::max<int64_t>(total_out, c.out_offset + c.nf);
+    }
+    plan->total_out = total_out;
+    refresh_meta_geometry(plan);
+    build_stage_list(plan);
+    return GLS_OK;
+}
+
+int gls_graph_update(gls_graph* graph, const gls_plan* plan, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!graph || !plan) return fail(GLS_E_INVALID, "graph or plan is NULL");
+    if (graph->plan_id != plan->id)
+        return fail(GLS_E_INVALID, "the graph was captured from another plan");
+    if (plan->meta.size() != graph->meta_bytes)
+        return fail(GLS_E_INVALID, "the plan's descriptors changed size since capture");
+    GLS_HIP_TRY(upload_async(graph->workspace, plan->meta.data(), plan->meta.size(),
+                             (hipStream_t)stream));
     return GLS_OK;
 }
 

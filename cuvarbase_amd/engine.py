@@ -33,7 +33,7 @@ class Plan:
 
     @classmethod
     def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method,
-             grid_chunk_bytes=0, window=0, window_beta=0.0):
+             grid_chunk_bytes=0, window=0, window_beta=0.0, point_capacity=0):
         _lib.init(0)
         n = len(curves)
         arr = (_lib.gls_curve * max(n, 1))()
@@ -43,10 +43,20 @@ class Plan:
                               reg_c=float(reg_c), reg_s=float(reg_s),
                               input_kind=int(input_kind), method=int(method),
                               window=int(window), window_beta=float(window_beta),
-                              grid_chunk_bytes=int(grid_chunk_bytes))
+                              grid_chunk_bytes=int(grid_chunk_bytes),
+                              point_capacity=int(point_capacity))
         h = C.c_void_p()
         _lib.call("gls_lomb_plan_create", C.byref(h), arr, C.c_int32(n), C.byref(prm))
         return cls(h.value)
+
+    def rebind(self, curves):
+        """Put other curves (same count and grids) on this plan (gls_plan_rebind)."""
+        n = len(curves)
+        arr = (_lib.gls_curve * max(n, 1))()
+        for i, c in enumerate(curves):
+            arr[i] = c
+        _lib.call("gls_plan_rebind", self.handle, arr, C.c_int32(n))
+        return self
 
     @classmethod
     def nfft(cls, items, sigma, m):
@@ -130,6 +140,11 @@ class Graph:
 
     def launch(self, stream):
         _lib.call("gls_graph_launch", self.handle, _handle(stream))
+
+    def update(self, plan, stream):
+        """Upload a rebound plan's descriptors into the graph's workspace on
+        ``stream`` (gls_graph_update); the next launch computes its curves."""
+        _lib.call("gls_graph_update", self.handle, plan.handle, _handle(stream))
 
     def destroy(self):
         h, self.handle = getattr(self, "handle", None), None
@@ -337,19 +352,28 @@ class NFFTBatch:
         self._y = np.concatenate([np.asarray(i[1], np.float64) for i in items]) if items else \
             np.zeros(0)
 
-    def run(self, stream=None):
-        s = stream or Stream()
-        d_t = DeviceArray.to_device(self._t if len(self._t) else np.zeros(1), stream=s)
-        d_y = DeviceArray.to_device(self._y if len(self._y) else np.zeros(1), stream=s)
+    def launch(self, stream):
+        """Enqueue upload, execute and download on ``stream``; returns the
+        page-locked complex64 result (valid once the stream has completed) and
+        the device buffers the enqueued work uses, to be kept alive until then."""
+        d_t = DeviceArray.to_device(self._t if len(self._t) else np.zeros(1), stream=stream)
+        d_y = DeviceArray.to_device(self._y if len(self._y) else np.zeros(1), stream=stream)
         d_out = DeviceArray(max(2 * self.total_out, 1), np.float32)
         d_ws = DeviceArray(self.plan.workspace_bytes, np.uint8)
         _lib.call("gls_nfft_execute", self.plan.handle, d_t.ptr, d_y.ptr, C.c_int32(0),
-                  d_out.ptr, d_ws.ptr, s.handle)
+                  d_out.ptr, d_ws.ptr, _handle(stream))
         out = pinned_empty(max(2 * self.total_out, 1), np.float32)
-        d_out.get_async(out, stream=s, count=2 * self.total_out)
+        d_out.get_async(out, stream=stream, count=2 * self.total_out)
+        return out[: 2 * self.total_out].view(np.complex64), (d_t, d_y, d_out, d_ws, self.plan)
+
+    def split(self, flat):
+        return [flat[o:o + n] for o, n in zip(self.out_offsets, self.nf)]
+
+    def run(self, stream=None):
+        s = stream or Stream()
+        flat, keep = self.launch(s)
         s.synchronize()
-        c = out[: 2 * self.total_out].view(np.complex64)
-        return [np.array(c[o:o + n]) for o, n in zip(self.out_offsets, self.nf)]
+        return [np.array(c) for c in self.split(flat)]
 
 
 def gaussian_grid(t, y, nf, sigma, m, samples_per_peak=1.0, stream=None):

@@ -26,13 +26,13 @@ import ctypes as C
 import gc
 import itertools
 import time
-from collections import deque
+from collections import OrderedDict, deque
 
 import numpy as np
 
 from . import _lib
 from ._device import DeviceArray, Event, Stream, pinned_empty, _handle
-from .engine import Plan, linear_grid
+from .engine import Graph, Plan, linear_grid
 from .tuning import choose_nfft_config
 from . import wire
 
@@ -148,16 +148,36 @@ class Request:
         self.arrival = time.perf_counter() if arrival is None else float(arrival)
         self.done_at = None
         self.power = None
-        self.top_power = self.top_index = self.top_freq = self.fap = None
+        self.top_power = self.top_index = None
+        self._top_freq = self._fap = None
         self._hold = None
+
+    @property
+    def top_freq(self):
+        """Frequencies of the top-k peaks (NaN where fewer than k exist)."""
+        if self._top_freq is None and self.top_index is not None:
+            self._top_freq = self.frequency(self.top_index)
+        return self._top_freq
+
+    @property
+    def fap(self):
+        """Baluev false-alarm probability of each top-k peak, computed on first
+        access (outside the serving loop)."""
+        if self._fap is None and self.top_power is not None:
+            self._fap = _fap(self, self.top_power, self.top_index)
+        return self._fap
 
     def release(self):
         """Give a zero-copy result's host memory back to the server; the
         result arrays must not be used afterwards."""
         h, self._hold = self._hold, None
         if h is not None:
+            if self.top_power is not None:
+                # keep what the peaks resolve to before their memory goes back
+                _ = self.top_freq, self.fap
+                self.top_power = np.array(self.top_power)
+                self.top_index = np.array(self.top_index)
             self.power = None
-            self.top_power = self.top_index = None
             h.drop()
 
     @property
@@ -184,6 +204,7 @@ class _Slot:
         self.h_freq = None    # pinned f64 (custom frequency lists)
         self.h_idx = None     # pinned i32 (top-k indices)
         self.batch = None
+        self.graphs = OrderedDict()   # repeated batch shape -> _GraphEntry
 
     def staging(self, nobs, nout, nfreq=0, nidx=0):
         # page-locked staging grows geometrically and is kept for the slot's life
@@ -214,6 +235,18 @@ class _Batch:
         self.t_launch = time.perf_counter()
 
 
+class _GraphEntry:
+    """A captured execute of one repeated batch shape on one slot, with the
+    device blocks it reads and writes at fixed addresses: the inputs (rows of
+    t, y, dy, each ``cap`` points long), the powers, the workspace and the
+    top-k output.  Later batches of the shape rebind the plan to their curves
+    and replay the graph."""
+
+    def __init__(self, plan, graph, blocks, cap):
+        self.plan, self.graph, self.blocks, self.cap = plan, graph, blocks, cap
+        self.uses = 0
+
+
 class GLSServer:
     """Continuous-batching server.
 
@@ -227,6 +260,12 @@ class GLSServer:
         single larger request still runs, growing its slot's staging)
     sigma, m, window, window_beta : NFFT parameters (default: tuned for
         |dP| <= ``tol``)
+    graph_batches : batches whose requests share one linear frequency grid
+        (and have at most 16,384 points each) run as replays of a captured
+        execute kept per (shape, slot) -- the plan is rebound to the new
+        curves and only its descriptors and the inputs are uploaded; at most
+        ``graphs_per_slot`` shapes are kept per slot (least recently used
+        first out), their device blocks held in the arena meanwhile
     zero_copy : results are views into a page-locked result arena of
         ``result_buffer_bytes`` (default 4 B x max_batch_evals x (slots + 1))
         that the D2H copies write directly; each request's memory returns to
@@ -237,7 +276,7 @@ class GLSServer:
     def __init__(self, hbm_budget=16e9, slots=3, max_batch_evals=3e8, max_batch_curves=8192,
                  sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN, zero_copy=False,
                  result_buffer_bytes=None, window=None, window_beta=0.0,
-                 max_batch_points=1 << 20):
+                 max_batch_points=1 << 20, graph_batches=True, graphs_per_slot=2):
         _lib.init(0)
         cfg = choose_nfft_config(tol)
         if sigma is not None or m is not None or window is not None:
@@ -253,6 +292,10 @@ class GLSServer:
         self.max_batch_evals = int(max_batch_evals)
         self.max_batch_curves = int(max_batch_curves)
         self.max_batch_points = max(int(max_batch_points), 1)
+        self.graph_batches = bool(graph_batches)
+        self.graphs_per_slot = max(int(graphs_per_slot), 1)
+        self.graph_captures = 0
+        self.graph_replays = 0
         self.arena = DeviceArena(hbm_budget)
         self.zero_copy = bool(zero_copy)
         self.host_arena = None
@@ -281,9 +324,13 @@ class GLSServer:
             # a budget too small for even this request: start cold
             self.queue.clear()
         r.release()
+        for slot in self.slots:
+            for key in list(slot.graphs):
+                self._drop_graph(slot, key)
         self.completed.clear()
         self.batches_launched = 0
         self.evals_done = 0
+        self.graph_captures = self.graph_replays = 0
 
     # -- client side ---------------------------------------------------------
     def submit(self, t, y, dy, freqs, arrival=None, custom_freqs=False, k=0, mode=None,
@@ -317,8 +364,6 @@ class GLSServer:
                 if hold is None:
                     r.top_power = np.array(r.top_power)
                     r.top_index = np.array(r.top_index)
-                r.top_freq = r.frequency(r.top_index)
-                r.fap = _fap(r, r.top_power, r.top_index)
             else:
                 r.power = out[o:o + r.nf] if hold is not None else np.array(out[o:o + r.nf])
             r._hold = hold
@@ -327,7 +372,8 @@ class GLSServer:
             self.evals_done += r.nf
         for blk in b.blocks:
             self.arena.release(blk)
-        b.plan.destroy()
+        if b.plan is not None:
+            b.plan.destroy()
         slot.batch = None
 
     def _class(self, r):
@@ -353,6 +399,67 @@ class GLSServer:
         self.queue.extendleft(reversed(rest))
         return reqs
 
+    def _graph_key(self, reqs, mode, k):
+        """Key of a repeatable batch shape, or None: every request on one
+        linear grid and within the one-block preparation limit."""
+        if not self.graph_batches:
+            return None
+        r0 = reqs[0]
+        if r0.freqs is not None:
+            return None
+        grid = (r0.f0, r0.df, r0.nf)
+        for r in reqs:
+            if r.freqs is not None or (r.f0, r.df, r.nf) != grid or len(r.t) > 16384:
+                return None
+        return (mode, k, grid, len(reqs))
+
+    def _drop_graph(self, slot, key):
+        entry = slot.graphs.pop(key)
+        entry.graph.destroy()
+        entry.plan.destroy()
+        for blk in entry.blocks:
+            self.arena.release(blk)
+
+    def _graph_entry(self, slot, key, curves, obs):
+        """The slot's captured execute for ``key``, bound to ``curves``
+        (captured on first use), or None if the arena cannot hold a new one
+        or the batch does not fit the captured one."""
+        entry = slot.graphs.get(key)
+        if entry is not None:
+            try:
+                entry.plan.rebind(curves)
+            except _lib.GLSError:
+                return None
+            slot.graphs.move_to_end(key)
+            return entry
+        mode, k, (f0, df, nf), nreq = key
+        cap = max(self.max_batch_points, obs)
+        plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
+                         _lib.METHOD_NFFT, window=self.window, window_beta=self.window_beta,
+                         point_capacity=cap)
+        while len(slot.graphs) >= self.graphs_per_slot:
+            self._drop_graph(slot, next(iter(slot.graphs)))
+        sizes = [24 * cap, 4 * nreq * nf, plan.workspace_bytes] + ([8 * nreq * k] if k else [])
+        blocks = []
+        for sz in sizes:
+            blk = self.arena.alloc(sz)
+            if blk is None:
+                for b in blocks:
+                    self.arena.release(b)
+                plan.destroy()
+                return None
+            blocks.append(blk)
+        p_in = self.arena.ptr(blocks[0])
+        row = 8 * cap
+        g = C.c_void_p()
+        _lib.call("gls_plan_graph_create", plan.handle, p_in, p_in + row, p_in + 2 * row,
+                  self.arena.ptr(blocks[1]), C.c_int32(0), self.arena.ptr(blocks[2]),
+                  slot.stream.handle, C.byref(g))
+        entry = _GraphEntry(plan, Graph(g.value), blocks, cap)
+        slot.graphs[key] = entry
+        self.graph_captures += 1
+        return entry
+
     def _launch(self, slot, reqs):
         custom, mode, topk = self._class(reqs[0])
         curves, obs, out, offsets = [], 0, 0, []
@@ -362,37 +469,45 @@ class GLSServer:
             offsets.append(out)
             obs += len(r.t)
             out += r.nf
-        plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
-                         _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT,
-                         window=self.window, window_beta=self.window_beta)
         k = max(r.k for r in reqs) if topk else 0
         nreq = len(reqs)
-        sizes = [24 * max(obs, 1), 4 * max(out, 1), plan.workspace_bytes]
-        if custom:
-            sizes.append(8 * max(out, 1))
-        if k:
-            sizes.append(8 * nreq * k)
-        blocks = []
-        for sz in sizes:
-            blk = self.arena.alloc(sz)
-            if blk is None:
-                for b in blocks:
-                    self.arena.release(b)
-                plan.destroy()
-                return False
-            blocks.append(blk)
+        key = None if custom else self._graph_key(reqs, mode, k)
+        entry = self._graph_entry(slot, key, curves, obs) if key is not None else None
+        if entry is not None:
+            # a replay: the entry's blocks stay with it after the batch
+            plan, blocks, dev = entry.plan, [], entry.blocks
+            row = 8 * entry.cap
+        else:
+            plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
+                             _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT,
+                             window=self.window, window_beta=self.window_beta)
+            sizes = [24 * max(obs, 1), 4 * max(out, 1), plan.workspace_bytes]
+            sizes.append(8 * nreq * k if k else 0)
+            if custom:
+                sizes.append(8 * max(out, 1))
+            blocks = []
+            for sz in sizes:
+                blk = self.arena.alloc(sz)
+                if blk is None:
+                    for b in blocks:
+                        self.arena.release(b)
+                    plan.destroy()
+                    return False
+                blocks.append(blk)
+            dev = blocks
+            row = 8 * max(obs, 1)
         hblock = None
         if self.zero_copy:
             hblock = self.host_arena.alloc(8 * nreq * k if k else 4 * max(out, 1))
             if hblock is None:
                 for b in blocks:
                     self.arena.release(b)
-                plan.destroy()
+                if entry is None:
+                    plan.destroy()
                 return False
         h_in, h_out = slot.staging(obs, 0 if self.zero_copy else (nreq * k if k else out),
                                    out if custom else 0, 0 if self.zero_copy else nreq * k)
-        # rows t, y, dy of the batch back to back, as the device block holds
-        # them: one copy
+        # rows t, y, dy of the batch back to back
         rows = h_in[:3 * obs].reshape(3, obs)
         o = 0
         for r in reqs:
@@ -402,24 +517,33 @@ class GLSServer:
             rows[2, o:o + n] = r.dy
             o += n
         s = slot.stream.handle
-        p_in = self.arena.ptr(blocks[0])
-        p_pow = self.arena.ptr(blocks[1])
-        p_ws = self.arena.ptr(blocks[2])
-        row = 8 * max(obs, 1)
-        _lib.call("gls_memcpy_h2d_async", p_in, h_in.ctypes.data, C.c_size_t(24 * obs), s)
-        if custom:
-            for r, o in zip(reqs, offsets):
-                slot.h_freq[o:o + r.nf] = r.freqs
-            p_fr = self.arena.ptr(blocks[3])
-            _lib.call("gls_memcpy_h2d_async", p_fr, slot.h_freq.ctypes.data,
-                      C.c_size_t(8 * out), s)
-            _lib.call("gls_lomb_execute_custom", plan.handle, p_in, p_in + row, p_in + 2 * row,
-                      p_fr, p_pow, C.c_int32(0), p_ws, s)
+        p_in, p_pow, p_ws = (self.arena.ptr(b) for b in dev[:3])
+        if entry is not None:
+            # rows land at the captured row stride: three copies; then the
+            # rebound descriptors, then the replay
+            for q in range(3):
+                _lib.call("gls_memcpy_h2d_async", p_in + q * row,
+                          h_in.ctypes.data + q * 8 * obs, C.c_size_t(8 * obs), s)
+            if entry.uses:
+                entry.graph.update(plan, slot.stream)
+            entry.graph.launch(slot.stream)
+            entry.uses += 1
+            self.graph_replays += 1
         else:
-            _lib.call("gls_lomb_execute", plan.handle, p_in, p_in + row, p_in + 2 * row,
-                      p_pow, p_ws, s)
+            _lib.call("gls_memcpy_h2d_async", p_in, h_in.ctypes.data, C.c_size_t(24 * obs), s)
+            if custom:
+                for r, o in zip(reqs, offsets):
+                    slot.h_freq[o:o + r.nf] = r.freqs
+                p_fr = self.arena.ptr(dev[4])
+                _lib.call("gls_memcpy_h2d_async", p_fr, slot.h_freq.ctypes.data,
+                          C.c_size_t(8 * out), s)
+                _lib.call("gls_lomb_execute_custom", plan.handle, p_in, p_in + row,
+                          p_in + 2 * row, p_fr, p_pow, C.c_int32(0), p_ws, s)
+            else:
+                _lib.call("gls_lomb_execute", plan.handle, p_in, p_in + row, p_in + 2 * row,
+                          p_pow, p_ws, s)
         if k:
-            p_top = self.arena.ptr(blocks[-1])
+            p_top = self.arena.ptr(dev[3])
             _lib.call("gls_topk", plan.handle, p_ws, p_pow, None, C.c_int32(k), p_top,
                       p_top + 4 * nreq * k, s)
             if hblock is not None:
@@ -434,7 +558,8 @@ class GLSServer:
             dst = self.host_arena.ptr(hblock) if hblock is not None else h_out.ctypes.data
             _lib.call("gls_memcpy_d2h_async", dst, p_pow, C.c_size_t(4 * out), s)
         slot.event.record(slot.stream)
-        slot.batch = _Batch(reqs, plan, blocks, obs, out, offsets, k=k, hblock=hblock)
+        slot.batch = _Batch(reqs, None if entry is not None else plan, blocks, obs, out,
+                            offsets, k=k, hblock=hblock)
         self.batches_launched += 1
         return True
 
@@ -495,11 +620,12 @@ class GLSServer:
                 gc.unfreeze()
 
     # -- measurement -------------------------------------------------------------
-    def serve_stream(self, arrivals, data, freqs, sleep=5e-5, release=False):
+    def serve_stream(self, arrivals, data, freqs, sleep=5e-5, release=False, k=0):
         """Replay a request stream in real time: request i arrives arrivals[i]
-        seconds after the start.  Returns a stats dict.  ``release=True``
-        hands each result back (zero-copy mode) as soon as it completes, as a
-        consumer that has taken it would."""
+        seconds after the start (asking for the k highest peaks when k > 0).
+        Returns a stats dict.  ``release=True`` hands each result back
+        (zero-copy mode) as soon as it completes, as a consumer that has
+        taken it would."""
         reqs = []
         i, n = 0, len(data)
         with self._gc_frozen():
@@ -508,7 +634,7 @@ class GLSServer:
                 now = time.perf_counter() - t0
                 while i < n and arrivals[i] <= now:
                     t, y, dy = data[i]
-                    reqs.append(self.submit(t, y, dy, freqs[i], arrival=t0 + arrivals[i]))
+                    reqs.append(self.submit(t, y, dy, freqs[i], arrival=t0 + arrivals[i], k=k))
                     i += 1
                 done = self.poll()
                 if release and done:
@@ -527,6 +653,8 @@ class GLSServer:
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "batches": self.batches_launched,
+            "graph_captures": self.graph_captures,
+            "graph_replays": self.graph_replays,
             "requests_obj": reqs,
         }
 

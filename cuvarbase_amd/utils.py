@@ -1,72 +1,57 @@
-"""Host utilities of the GLS path (cuvarbase/utils.py:10-146)."""
-from __future__ import annotations
+"""Host utilities of the GLS path: observation weights, the automatic
+frequency grid and the centring of light curves.
 
-from copy import deepcopy
+These keep the names, arguments and results of cuvarbase/utils.py:10-13
+(``weights``), :46-97 (``autofrequency``), :106-110 (``get_autofreqs``) and
+:113-146 (``normalize_light_curves``).
+"""
+from __future__ import annotations
 
 import numpy as np
 
+# the keyword arguments that shape the automatic grid
+GRID_KEYWORDS = ("minimum_frequency", "maximum_frequency", "nyquist_factor",
+                 "samples_per_peak")
+
 
 def weights(err):
-    """Observation weights from uncertainties, normalised to sum 1
-    (cuvarbase/utils.py:10-13)."""
-    w = np.power(err, -2)
-    return w / sum(w)
-
-
-def tophat_window(t, t0, d):
-    """cuvarbase/utils.py:35-38."""
-    w_window = np.zeros_like(t)
-    w_window[np.absolute(t - t0) < d] += 1.
-    return w_window / max(w_window)
-
-
-def gaussian_window(t, t0, d):
-    """cuvarbase/utils.py:41-43."""
-    w_window = np.exp(-0.5 * np.power(t - t0, 2) / (d * d))
-    return w_window / (1. if len(w_window) == 0 else max(w_window))
+    """Inverse-variance weights with unit sum: err_i^-2 / sum_j err_j^-2."""
+    inv_var = np.reciprocal(np.square(np.asarray(err, dtype=np.float64)))
+    return inv_var / np.sum(inv_var)
 
 
 def autofrequency(t, nyquist_factor=5, samples_per_peak=5,
                   minimum_frequency=None, maximum_frequency=None, **kwargs):
-    """Heuristic linear frequency grid ``df * (nf0 + arange(Nf))``
-    (cuvarbase/utils.py:46-97): df = 1/(baseline * samples_per_peak);
-    nf0 = max(1, int(minimum_frequency/df)); Nf = int(maximum_frequency/df) - nf0
-    or int(0.5 * samples_per_peak * nyquist_factor * len(t))."""
-    baseline = max(t) - min(t)
-    n_samples = len(t)
-    df = 1. / (baseline * samples_per_peak)
-    nf0 = 1
-    if minimum_frequency is not None:
-        nf0 = max([nf0, int(minimum_frequency / df)])
-    if maximum_frequency is not None:
-        Nf = int(maximum_frequency / df) - nf0
+    """Linear grid on multiples of df = 1 / (baseline * samples_per_peak).
+
+    The grid starts at the multiple ``max(1, int(minimum_frequency / df))``
+    (1 without a minimum) and holds ``int(maximum_frequency / df)`` minus that
+    many frequencies, or ``int(samples_per_peak * nyquist_factor * N / 2)``
+    without a maximum -- the "average Nyquist" heuristic of N points.
+    Extra keyword arguments are ignored."""
+    t = np.asarray(t)
+    df = 1.0 / ((np.max(t) - np.min(t)) * samples_per_peak)
+    first = 1 if minimum_frequency is None else max(1, int(minimum_frequency / df))
+    if maximum_frequency is None:
+        count = int(0.5 * samples_per_peak * nyquist_factor * len(t))
     else:
-        Nf = int(0.5 * samples_per_peak * nyquist_factor * n_samples)
-    return df * (nf0 + np.arange(Nf))
-
-
-def dphase(dt, freq):
-    """Phase distance in [0, 1/2] (cuvarbase/utils.py:100-103)."""
-    dph = dt * freq - np.floor(dt * freq)
-    return dph if dph < 0.5 else 1 - dph
+        count = int(maximum_frequency / df) - first
+    return df * np.arange(first, first + count)
 
 
 def get_autofreqs(t, **kwargs):
-    """autofrequency with only its own keyword arguments (cuvarbase/utils.py:106-110)."""
-    keys = ('minimum_frequency', 'maximum_frequency', 'nyquist_factor', 'samples_per_peak')
-    return autofrequency(t, **{k: v for k, v in kwargs.items() if k in keys})
+    """``autofrequency(t)`` driven by the grid keywords found in ``kwargs``."""
+    return autofrequency(t, **{k: kwargs[k] for k in GRID_KEYWORDS if k in kwargs})
 
 
 def normalize_light_curves(data):
-    """Copy each (t, y, ...) tuple and subtract the nanmean of t and y
-    (cuvarbase/utils.py:113-146).  The caller's arrays are not modified."""
-    data = deepcopy(data)
-    for i, lc in enumerate(data):
-        out = []
-        for j in range(len(lc)):
-            if j < 2:
-                out.append((np.asarray(lc[j]) - np.nanmean(lc[j])).copy())
-            else:
-                out.append(np.asarray(lc[j]).copy())
-        data[i] = tuple(out)
-    return data
+    """New tuples with t and y centred on their NaN-ignoring means; any further
+    columns (dy, ...) are copied unchanged.  The caller's arrays are never
+    modified."""
+    centred = []
+    for columns in data:
+        cols = [np.array(c, copy=True) for c in columns]
+        for j in range(min(2, len(cols))):
+            cols[j] = cols[j] - np.nanmean(cols[j])
+        centred.append(tuple(cols))
+    return centred

@@ -96,6 +96,10 @@ typedef struct gls_params {
                            shrinks from all grids of the batch to the largest chunk, at a
                            few per cent of throughput); <= 0: one region per curve.
                            No reference counterpart. */
+    int64_t point_capacity; /* observations the plan's per-point workspace holds; <= the
+                           batch's own total: exactly that total.  Larger capacities let
+                           gls_plan_rebind put other curves (more points) on the plan.
+                           No reference counterpart. */
 } gls_params;
 
 /* One adjoint-NFFT item: sum_j y_j exp(2 pi i (f0 + k df) t_j), k < nf. */
@@ -236,6 +240,21 @@ int gls_plan_graph_create(gls_plan* plan, const double* t, const void* a,
                           void* workspace, void* stream, gls_graph** graph);
 int gls_graph_launch(gls_graph* graph, void* stream);
 int gls_graph_destroy(gls_graph* graph);
+
+/* Repeated batch shapes (no reference counterpart; the reference allocates and
+ * plans every curve anew, cuvarbase/lombscargle.py:1065-1074).
+ * gls_plan_rebind replaces the curves of a Lomb-Scargle plan by `curves`:
+ * the same count, each with the frequency grid (nf, f0, df) of the curve it
+ * replaces, on the same side of the 16,384-point preparation limit, no more
+ * points in all than the plan's point_capacity (and wide curves no longer
+ * than the plan's longest); observation and output offsets, sizes and the
+ * weighted-input ybar/yy may change.  Host-side only: executes afterwards use
+ * the new curves.  gls_graph_update enqueues, on `stream`, the upload of the
+ * plan's descriptors into the workspace of a graph captured from that plan,
+ * so the next gls_graph_launch on the stream computes the rebound curves
+ * (from the graph's fixed input and output pointers). */
+int gls_plan_rebind(gls_plan* plan, const gls_curve* curves, int32_t ncurves);
+int gls_graph_update(gls_graph* graph, const gls_plan* plan, void* stream);
 
 /* Custom-frequency plans (GLS_METHOD_DIRECT_CUSTOM): curve i is evaluated
  * at freqs[out_offset .. out_offset + nf) (device f64; f0/df ignored) into

@@ -656,3 +656,93 @@ def test_captured_graph_replays_wide_curves(config):
         assert np.max(np.abs(got[0][:40] - Pd)) < TOL
         engine.LombBatch(data[1:2], freqs[1:2]).run()   # eager work between
     g.destroy()
+
+
+def test_nfft_use_double_returns_complex128():
+    """NFFTAsyncProcess(use_double=True) returns complex128 transforms
+    (cuvarbase/cunfft.py:147-160), batched and through allocated memory, equal
+    to the direct DFT within the FP32 pipeline's accuracy."""
+    from cuvarbase_amd.cunfft import NFFTAsyncProcess
+
+    rng = np.random.RandomState(100)
+    t = np.sort(rng.rand(200))
+    y = np.cos(2 * np.pi * 30.0 * t) + 0.1 * rng.randn(200)
+    nf = 400
+    df = 1.0 / (t.max() - t.min())
+    ref = on.direct_dft(t, y, df * np.arange(nf))
+    tol = 1e-5 * np.sum(np.abs(y))
+    single = NFFTAsyncProcess(sigma=4, m=8)
+    g32 = single.run([(t, y, nf)])[0]
+    single.finish()
+    double = NFFTAsyncProcess(sigma=4, m=8, use_double=True)
+    g64 = double.run([(t, y, nf)])[0]
+    double.finish()
+    mem = double.allocate([(t, y, nf)])
+    g64m = double.run([(t, y, nf)], memory=mem)[0]
+    double.finish()
+    assert np.asarray(g32).dtype == np.complex64
+    assert g64.dtype == np.complex128 and g64m.dtype == np.complex128
+    for g in (g32, g64, g64m):
+        assert np.max(np.abs(np.asarray(g) - ref)) < tol
+
+
+def test_function_tuples_hold_callables():
+    """function_tuple entries are callable launchers, or None for the
+    reference stages that run fused inside other launches."""
+    from cuvarbase_amd.cunfft import NFFTMemory
+    from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
+
+    proc = LombScargleAsyncProcess(sigma=5, m=8)
+    proc._compile_and_prepare_functions()
+    lomb, dirsum = proc.function_tuple
+    assert callable(lomb) and callable(dirsum)
+    nfft_funcs = proc.nfft_proc.function_tuple
+    assert [f is None for f in nfft_funcs] == [True, False, False, True, True]
+    rng = np.random.RandomState(100)
+    t = np.sort(rng.rand(100))
+    y = rng.randn(100)
+    mem = NFFTMemory(5, None, 8).fromdata(t, y, nf=500)
+    mem.transfer_data_to_gpu()
+    for grid_fn in nfft_funcs[1:3]:
+        g = grid_fn(mem)
+        np.testing.assert_allclose(g, on.gaussian_grid_reference(t, y, 500, 5, 8, 1.0),
+                                   atol=1e-5)
+
+
+def test_memory_requirement_is_a_real_plan():
+    from cuvarbase_amd import engine
+    from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
+
+    proc = LombScargleAsyncProcess(sigma=2, m=8)
+    req = proc.memory_requirement(1000, 10000, 10)
+    t, y, dy = _curve(7, n=1000)
+    freqs = 0.01 * (10 + np.arange(10000))
+    b = engine.LombBatch([(t, y, dy)], [freqs], sigma=2, m=8)
+    assert req == b.plan.workspace_bytes + 24 * 1000 + 4 * 10000
+    assert proc.memory_requirement(1000, 10000, 10, nbatch=3) > 2.9 * req
+    assert 0 < proc.memory_requirement(1000, 10000, 10, use_fft=False) < req
+    es = LombScargleAsyncProcess(sigma=1.35, m=5, nfft_window="es")
+    assert es.memory_requirement(1000, 10000, 10) < req
+
+
+def test_drop_in_es_window_meets_target():
+    """The drop-in with nfft_window="es" at the tuned sigma = 1.35, m = 5
+    (the plain path the benchmark times) against FP64 direct sums."""
+    from cuvarbase_amd import synthetic
+    from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
+
+    data, freqs = synthetic.survey(12, seed=31)
+    proc = LombScargleAsyncProcess(sigma=1.35, m=5, nfft_window="es")
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for d, f in zip(data, freqs):
+        (_, p), = proc.run([d], freqs=f)
+        proc.finish()
+        kb = int(np.argmax(p))
+        sel = np.unique(np.concatenate([np.arange(20), np.arange(max(kb - 10, 0), kb + 11),
+                                        rng.choice(len(f), 60, replace=False)]))
+        ref = og.gls_direct(*d, f[sel])
+        worst = max(worst, float(np.max(np.abs(np.asarray(p, np.float64)[sel] - ref))))
+        assert int(np.argmax(ref)) == int(np.argmax(np.asarray(p)[sel]))
+    print(f"drop-in ES sigma=1.35 m=5: max|dP| = {worst:.2e}")
+    assert worst < TOL

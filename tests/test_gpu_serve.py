@@ -144,3 +144,39 @@ def test_zero_copy_results_and_release():
         rb.release()
         assert rb.power is None
     assert b.host_arena.used == 0
+
+
+def test_graph_served_top5_matches_eager_and_oracle():
+    """The short-curve workload through the server on one shared grid, k = 5:
+    batches replayed from a captured execute (rebound plan, descriptors
+    uploaded per batch) against eager batches and against the FP64 oracle's
+    top 5 (ties within 1e-4 admitted), as in config 5."""
+    from oracle import gls as og
+
+    from cuvarbase_amd import synthetic
+    from cuvarbase_amd.serve import GLSServer
+
+    data, freqs = synthetic.short(ncurves=24)
+    grid = freqs[0]
+
+    def served(graphs):
+        srv = GLSServer(hbm_budget=30e9, slots=1, max_batch_curves=8, graph_batches=graphs)
+        reqs = [srv.submit(*d, grid, k=5) for d in data]
+        srv.drain()
+        return srv, reqs
+
+    sg, rg = served(True)
+    se, re_ = served(False)
+    assert sg.graph_captures == 1 and sg.graph_replays == 3
+    assert se.graph_captures == 0 and se.graph_replays == 0
+    for i, (a, b, d) in enumerate(zip(rg, re_, data)):
+        np.testing.assert_allclose(a.top_power, b.top_power, atol=MIX_TOL)
+        Po = og.gls_direct(*d, grid)
+        order = np.argsort(-Po, kind="stable")[:5]
+        for q in range(5):
+            j = int(a.top_index[q])
+            assert abs(float(a.top_power[q]) - Po[j]) <= 1e-4, (i, q)
+            assert j == int(order[q]) or abs(Po[j] - Po[order[q]]) <= 1e-4, (i, q, j, order)
+            jb = int(b.top_index[q])
+            assert j == jb or abs(Po[j] - Po[jb]) <= 1e-4
+        assert np.all((a.fap >= 0) & (a.fap <= 1))

@@ -84,9 +84,10 @@ def test_process_construction_without_device():
     p = ls.LombScargleAsyncProcess(autoset_m=True, sigma=4, tol=1e-8)
     assert p.nfft_proc.get_m(10000) == int(np.ceil(-np.log(0.25 * 1e-12) /
                                                    (np.pi * (1 - 1 / 7.0))))
-    req = proc.memory_requirement(1000, 10000, 10)
-    assert req > proc.memory_requirement(1000, 10000, 10, use_fft=False) > 0
-    assert proc.memory_requirement(1000, 10000, 10, nbatch=3) == 3 * req
+    # window selection is validated at construction, without a device
+    with pytest.raises(ValueError):
+        ls.LombScargleAsyncProcess(nfft_window="kaiser")
+    assert ls.LombScargleAsyncProcess(nfft_window="es", sigma=1.35, m=5).nfft_window == "es"
 
 
 def test_synthetic_workloads_are_seeded():

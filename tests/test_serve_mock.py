@@ -23,7 +23,9 @@ class FakeDevice:
     def __init__(self, seed=1):
         self.rng = random.Random(seed)
         self.keep, self.plans, self.streams, self.events = {}, {}, {}, {}
+        self.graphs, self.ws_meta = {}, {}
         self.next_plan = 1000
+        self.next_graph = 5000
 
     def _enq(self, s, fn):
         self.streams.setdefault(s, []).append(fn)
@@ -101,26 +103,64 @@ class FakeDevice:
                 fr = None
             else:
                 h, t, y, dy, fr, pw, _f64, ws, s = a
+            # an execute uploads the plan's descriptors (as they are at the
+            # call) into the workspace, then computes from them
+            meta = list(self.plans[h]["curves"])
+            mode = self.plans[h]["mode"]
 
             def run():
-                P = self.plans[h]
-                assert P["alive"], "plan destroyed before its execute ran"
+                assert self.plans[h]["alive"], "plan destroyed before its execute ran"
                 C.memset(ws, 0x7F, 4096)   # a shared workspace would be clobbered
-                for (o, n, nf, f0, df, oo) in P["curves"]:
-                    tt = self._arr(C.c_double, t + 8 * o, n).copy()
-                    yy = self._arr(C.c_double, y + 8 * o, n).copy()
-                    dd = self._arr(C.c_double, dy + 8 * o, n).copy()
-                    f = (f0 + df * np.arange(nf) if fr is None
-                         else self._arr(C.c_double, fr + 8 * oo, nf).copy())
-                    self._arr(C.c_float, pw + 4 * oo, nf)[:] = og.gls_direct(
-                        tt, yy, dd, f, mode=P["mode"])
+                self.ws_meta[ws] = meta
+                self._compute(meta, mode, t, y, dy, fr, pw)
             self._enq(s, run)
+            return 0
+        if name == "gls_plan_graph_create":
+            h, t, y, dy, pw, _f64, ws, s, out = args
+            h, t, y, dy, pw, ws, s = (x.value if isinstance(x, C.c_void_p) else x
+                                      for x in (h, t, y, dy, pw, ws, s))
+            self._run(s)   # creation synchronises its stream
+            self.ws_meta[ws] = list(self.plans[h]["curves"])
+            self.next_graph += 1
+            self.graphs[self.next_graph] = dict(plan=h, ptrs=(t, y, dy, pw), ws=ws, alive=True,
+                                                mode=self.plans[h]["mode"])
+            out._obj.value = self.next_graph
+            return 0
+        if name == "gls_plan_rebind":
+            h, arr, n = args
+            h = h.value if isinstance(h, C.c_void_p) else h
+            P = self.plans[h]
+            assert P["alive"] and n.value == len(P["curves"])
+            new = [(c.obs_offset, c.n_obs, c.nf, c.f0, c.df, c.out_offset)
+                   for c in (arr[i] for i in range(n.value))]
+            assert [c[2:5] for c in new] == [c[2:5] for c in P["curves"]], "grids changed"
+            P["curves"] = new
+            return 0
+        if name == "gls_graph_update":
+            g, h, s = a
+            G = self.graphs[g]
+            assert G["alive"] and G["plan"] == h
+            meta = list(self.plans[h]["curves"])
+            self._enq(s, lambda: self.ws_meta.__setitem__(G["ws"], meta))
+            return 0
+        if name == "gls_graph_launch":
+            g, s = a
+            G = self.graphs[g]
+
+            def replay():
+                assert G["alive"], "graph destroyed before its replay ran"
+                t, y, dy, pw = G["ptrs"]
+                self._compute(self.ws_meta[G["ws"]], G["mode"], t, y, dy, None, pw)
+            self._enq(s, replay)
+            return 0
+        if name == "gls_graph_destroy":
+            self.graphs[a[0]]["alive"] = False
             return 0
         if name == "gls_topk":
             h, ws, pw, mask, k, tp, ti, s = a
 
             def run():
-                for ci, (o, n, nf, f0, df, oo) in enumerate(self.plans[h]["curves"]):
+                for ci, (o, n, nf, f0, df, oo) in enumerate(self.ws_meta[ws]):
                     p = self._arr(C.c_float, pw + 4 * oo, nf).copy()
                     order = np.argsort(-p, kind="stable")[:k]
                     self._arr(C.c_float, tp + 4 * ci * k, k)[:] = p[order]
@@ -128,6 +168,15 @@ class FakeDevice:
             self._enq(s, run)
             return 0
         raise AssertionError(f"unexpected libgls call {name}")
+
+    def _compute(self, meta, mode, t, y, dy, fr, pw):
+        for (o, n, nf, f0, df, oo) in meta:
+            tt = self._arr(C.c_double, t + 8 * o, n).copy()
+            yy = self._arr(C.c_double, y + 8 * o, n).copy()
+            dd = self._arr(C.c_double, dy + 8 * o, n).copy()
+            f = (f0 + df * np.arange(nf) if fr is None
+                 else self._arr(C.c_double, fr + 8 * oo, nf).copy())
+            self._arr(C.c_float, pw + 4 * oo, nf)[:] = og.gls_direct(tt, yy, dd, f, mode=mode)
 
 
 @pytest.fixture
@@ -254,3 +303,50 @@ def test_tiny_budget_skips_warm_up(serve):
     srv.submit(t, y, dy, freqs[0])
     with pytest.raises(MemoryError, match="budget"):
         srv.drain()
+
+
+def test_shared_grid_batches_replay_captured_graphs(serve):
+    """Requests on one linear grid: the first batch of a shape captures a
+    graph on its slot, later batches of the shape rebind the plan and replay
+    it (inputs at the captured row stride, descriptors uploaded before the
+    replay); top-k and whole periodograms match the oracle, and a batch of
+    another size is a new shape."""
+    rng = np.random.default_rng(11)
+    grid = 0.05 + 0.01 * np.arange(300)
+    data = []
+    for i in range(14):
+        m = int(rng.integers(20, 90))
+        t = np.sort(rng.uniform(0, 20, m))
+        data.append((t, np.cos(t * (1.5 + 0.3 * i)) + rng.normal(0, 0.3, m), np.full(m, 0.3)))
+    for k in (0, 4):
+        srv = serve.GLSServer(hbm_budget=1 << 24, slots=1, max_batch_curves=4,
+                              max_batch_points=400)
+        reqs = [srv.submit(*d, grid, k=k) for d in data]
+        srv.drain()
+        assert srv.batches_launched == 4          # 4 + 4 + 4 + 2 curves
+        assert srv.graph_captures == 2 and srv.graph_replays == 4
+        for r, d in zip(reqs, data):
+            ref = og.gls_direct(*d, grid)
+            if k:
+                order = np.argsort(-ref.astype(np.float32), kind="stable")[:k]
+                np.testing.assert_array_equal(r.top_index, order)
+                np.testing.assert_allclose(r.top_power, ref[order], atol=1e-6)
+            else:
+                np.testing.assert_allclose(r.power, ref, atol=1e-6)
+
+
+def test_graph_entries_are_bounded_per_slot(serve):
+    rng = np.random.default_rng(12)
+    srv = serve.GLSServer(hbm_budget=1 << 24, slots=1, max_batch_curves=2,
+                          max_batch_points=300, graphs_per_slot=2)
+    used_before = srv.arena.used
+    for nf in (200, 210, 220, 200):
+        grid = 0.05 + 0.01 * np.arange(nf)
+        for _ in range(2):
+            m = int(rng.integers(20, 60))
+            t = np.sort(rng.uniform(0, 20, m))
+            srv.submit(t, np.sin(t) + rng.normal(0, 0.2, m), np.full(m, 0.2), grid)
+        srv.drain()
+    assert len(srv.slots[0].graphs) == 2 and srv.graph_captures == 4
+    assert srv.arena.used > used_before
+    assert all(r.power is not None for r in srv.completed)
