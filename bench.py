@@ -39,6 +39,18 @@ def _cpu_worker(args):
     return len(freqs), (P if keep else None)
 
 
+def long_curve_oracle():
+    """FP64 oracle periodogram of the long curve (200,000 points, ~1.8M
+    frequencies; Gaussian sigma = 2, m = 10 NFFT, |dP| ~1e-9): the reference
+    of the precision study, computed by the CPU-baseline leg."""
+    from cuvarbase_amd import synthetic
+    from oracle import nfft as onfft
+
+    data, freqs = synthetic.long()
+    t, y, dy = data[0]
+    return onfft.gls_nfft(t, y, dy, freqs[0], sigma=2.0, m=10)
+
+
 def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8):
     """Time the FP64 oracle port (numpy NFFT + GLS) on a bounded sample.
 
@@ -137,6 +149,29 @@ def _parity(gpu, oracle):
             "tolerance": TOL,
             "oracle": "FP64 NumPy NFFT sigma=2 m=8 + GLS (the CPU-baseline leg's own outputs "
                       "on its first sampled curves)"}
+
+
+def precision_study(data, freqs, batch, oracle_P, sigma, m, win, stream):
+    """max |dP| and best period against the FP64 oracle of the long curve:
+    FP32 with FP64 time/cell/phase reduction (``batch``, the default), and
+    FP32 everywhere (the reference's FP32 time arithmetic, same kernels)."""
+    from cuvarbase_amd import _lib, engine
+
+    f = freqs[0]
+    kbest = int(np.argmax(oracle_P))
+    out = {"oracle": "FP64 NumPy NFFT (Gaussian sigma=2, m=10) + FP64 GLS, computed in the "
+                     "CPU-baseline leg", "oracle_best_period_d": float(1 / f[kbest])}
+    fp32 = engine.LombBatch(data, freqs, sigma=sigma, m=m, precision=_lib.PRECISION_FP32, **win)
+    for name, b in (("fp32_with_fp64_phase", batch), ("fp32_everywhere", fp32)):
+        b.upload(stream=stream)
+        b.execute(stream=stream)
+        P = b.split(b.download_async(stream=stream))[0]
+        stream.synchronize()
+        P = np.asarray(P, dtype=np.float64)
+        k = int(np.argmax(P))
+        out[name] = {"max_abs_dP": float(np.max(np.abs(P - oracle_P))),
+                     "best_period_d": float(1 / f[k]), "same_best_index": k == kbest}
+    return out
 
 
 def end_to_end(batch, stream, reps=3, k=5):
@@ -245,6 +280,7 @@ def main():
 
     cpu, oracle_P = (None, {}) if args.no_cpu else cpu_baseline(data, freqs,
                                                                  keep=args.parity_sample)
+    long_P = None if (args.no_cpu or not args.shapes) else long_curve_oracle()
 
     from cuvarbase_amd import engine, _device
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
@@ -476,6 +512,9 @@ def main():
             shapes[name] = {"points": len(cd[0][0]), "evals": int(cb.total_out),
                             "ms": cms, "evals_per_s": cb.total_out / (cms / 1e3),
                             "graph_ms": gms}
+            if name.startswith("long") and long_P is not None:
+                shapes[name]["precision_study"] = precision_study(cd, cf, cb, long_P, sigma,
+                                                                  m, win, s)
             del cb
         _device.synchronize()
 

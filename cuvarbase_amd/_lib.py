@@ -35,6 +35,9 @@ METHOD_DIRECT_CUSTOM = 2
 WINDOW_GAUSSIAN = 0
 WINDOW_ES = 1
 
+PRECISION_FP64_COORDS = 0
+PRECISION_FP32 = 1
+
 STATIC_SCRATCH_BYTES = 128 << 20   # GLS_STATIC_SCRATCH_BYTES
 
 
@@ -65,7 +68,8 @@ class gls_params(C.Structure):
                 ("reg_c", C.c_double), ("reg_s", C.c_double),
                 ("input_kind", C.c_int32), ("method", C.c_int32),
                 ("window", C.c_int32), ("window_beta", C.c_float),
-                ("grid_chunk_bytes", C.c_int64), ("point_capacity", C.c_int64)]
+                ("grid_chunk_bytes", C.c_int64), ("point_capacity", C.c_int64),
+                ("precision", C.c_int32), ("pad", C.c_int32)]
 
 
 class gls_mh_curve(C.Structure):

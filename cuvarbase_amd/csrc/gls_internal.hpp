@@ -90,6 +90,7 @@ struct CurveState {
     double ybar;
     double yy;
     double tspan;      // max t - min t
+    double tmean;      // mean of the finite t (FP32-coordinate plans only)
 };
 
 // per-point workspace (structure of arrays, indexed by CurveGeom::pts + j)
@@ -112,6 +113,7 @@ struct PlanParams {
     int32_t kind;      // 0 = lomb, 1 = nfft
     int32_t window;    // GLS_WINDOW_*
     float es_beta;     // ES shape parameter
+    int32_t fp32_coords;  // GLS_PRECISION_FP32: FP32 time offsets, cells and phases
     // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (the
     // library's resident device copy for (m, beta), es_table_device)
     const float* es_deconv;
@@ -154,19 +156,34 @@ constexpr int kLowSplit = 8;
 struct WideAux {
     double ymean;
     double s0inv;
-    int32_t done;
+    int32_t done;       // counting blocks finished
+    int32_t red_done;   // reduction blocks finished
+    int32_t yy_done;    // scatter blocks finished (variance partials)
     int32_t pad;
     int32_t hist[2][kMaxBuckets];
     double low[kLowMax][6];
     int32_t low_done[kLowMax];
 };
+// Per-chunk partial reductions of a wide curve, one slot per chunk of
+// kWideChunk points, combined in chunk order by the last block to finish (so
+// the result does not depend on block timing).
+struct WidePart {
+    double tmin, tmax;   // time range
+    double st, ct;       // sum and count of the finite t (FP32 coordinates only)
+    double sy, cy;       // sum and count of the finite y (raw input)
+    double sw, swy;      // sum of 1/dy^2 and of y/dy^2 (raw input)
+    double syy;          // sum of (y - ymean - ybar)^2/dy^2 (raw input, from the scatter)
+    double pad;
+};
 // aux: one WideAux per wide curve (NFFT method), or nullptr to prepare each
-// wide curve in one block; wide_max_obs: the largest n_obs in the wide list
+// wide curve in one block; part: ceil(wide_max_obs / kWideChunk) WidePart
+// slots per wide curve; wide_max_obs: the largest n_obs in the wide list
 hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                        const int32_t* wide, int nwide, CurveState* state,
                        const double* t, const void* a, const void* b,
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
-                       int y_is_f32, WideAux* aux, int wide_max_obs, hipStream_t s);
+                       int y_is_f32, WideAux* aux, WidePart* part, int wide_max_obs,
+                       hipStream_t s);
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
 hipError_t launch_pass1(int n1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,

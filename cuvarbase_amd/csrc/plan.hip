@@ -52,8 +52,9 @@ struct gls_plan {
     size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
            off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
-    // per wide curve bucketing records (NFFT method; see WideAux)
-    size_t off_wide = 0;
+    // per wide curve bucketing records (NFFT method; see WideAux) and chunk
+    // partials of their reductions (WidePart)
+    size_t off_wide = 0, off_wpart = 0;
     int nwide_aux = 0;
     int wide_max_obs = 0;
     size_t ws_bytes = 0;
@@ -219,7 +220,11 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
             ++P->nwide_aux;
             P->wide_max_obs = std::max(P->wide_max_obs, (int)G.n_obs);
         }
-    if (P->nwide_aux > 0) P->off_wide = L.take(sizeof(WideAux) * (size_t)P->nwide_aux);
+    if (P->nwide_aux > 0) {
+        P->off_wide = L.take(sizeof(WideAux) * (size_t)P->nwide_aux);
+        const size_t nchunks = ((size_t)P->wide_max_obs + kWideChunk - 1) / kWideChunk;
+        P->off_wpart = L.take(sizeof(WidePart) * nchunks * (size_t)P->nwide_aux);
+    }
     P->off_grid = L.take(sizeof(float2) * grid_elems);
     P->bcurves = bcurves;
     // lanes: buckets own disjoint grid regions unless chunked, so they may run
@@ -374,7 +379,7 @@ void build_stage_list(gls_plan* P) {
             P->stages.push_back({1, B.n1 << B.p2, b1});
             P->stages.push_back({2, B.n1 << B.p2, b2});
         }
-        if (P->kind == 0) {
+        if (P->kind == 0 && !P->pp.fp32_coords) {
             double bl = 0.0;
             for (auto& G : P->geom) bl += 16.0 * G.n_obs;
             P->stages.push_back({4, 0, bl});
@@ -477,9 +482,10 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
     GLS_HIP_TRY(mark());
     const int32_t* d_prep = (const int32_t*)(base + P->off_prep);
     WideAux* aux = P->nwide_aux > 0 ? (WideAux*)(base + P->off_wide) : nullptr;
+    WidePart* wpart = P->nwide_aux > 0 ? (WidePart*)(base + P->off_wpart) : nullptr;
     GLS_HIP_TRY(launch_prep(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
                             P->ncurves - P->nprep_small, state, t, a, b, pa, bkt, pp, y_is_f32,
-                            aux, P->wide_max_obs, s));
+                            aux, wpart, P->wide_max_obs, s));
     GLS_HIP_TRY(mark());
     if (P->pp.method == GLS_METHOD_NFFT) {
         // fork the buckets over the caller's and the auxiliary streams (not
@@ -519,7 +525,7 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
                 GLS_HIP_TRY(hipStreamWaitEvent(s, P->fork_join[j], 0));
             }
         }
-        if (P->kind == 0) {
+        if (P->kind == 0 && !P->pp.fp32_coords) {
             GLS_HIP_TRY(launch_lowfreq(d_geom, d_prep, P->nprep_small, d_prep + P->nprep_small,
                                        P->ncurves - P->nprep_small, state, pa, pp, out,
                                        out_f64, aux, s));
@@ -598,6 +604,10 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         return fail(GLS_E_INVALID, "window_beta must be finite and <= 200");
     if (prm.point_capacity > ((int64_t)1 << 34))
         return fail(GLS_E_INVALID, "point_capacity above 2^34");
+    if (prm.precision != GLS_PRECISION_FP64_COORDS && prm.precision != GLS_PRECISION_FP32)
+        return fail(GLS_E_INVALID, "unknown precision");
+    if (prm.precision == GLS_PRECISION_FP32 && prm.method != GLS_METHOD_NFFT)
+        return fail(GLS_E_INVALID, "GLS_PRECISION_FP32 is an NFFT-method option");
     gls_plan* P = new gls_plan();
     P->id = g_plan_ids.fetch_add(1);
     P->kind = 0;
@@ -609,6 +619,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.method = prm.method;
     P->pp.kind = 0;
     P->pp.window = (prm.method == GLS_METHOD_NFFT) ? prm.window : GLS_WINDOW_GAUSSIAN;
+    P->pp.fp32_coords = prm.precision == GLS_PRECISION_FP32 ? 1 : 0;
     P->pp.es_beta = prm.window_beta > 0.f
                         ? prm.window_beta
                         : (float)(M_PI * (2 * prm.m + 1) * (1.0 - 0.5 / prm.sigma));
@@ -671,7 +682,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         const size_t shift = L.cur;
         for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_v1, &P->off_U2,
                           &P->off_fr2, &P->off_v2, &P->off_w, &P->off_yw, &P->off_dt,
-                          &P->off_bkt, &P->off_wide, &P->off_grid})
+                          &P->off_bkt, &P->off_wide, &P->off_wpart, &P->off_grid})
             *o += shift;
         L.cur += probe.cur;
     } else {
@@ -743,7 +754,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     P->off_state = L.take(sizeof(CurveState) * (size_t)nitems);
     const size_t shift = L.cur;
     for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_bkt, &P->off_wide,
-                      &P->off_grid})
+                      &P->off_wpart, &P->off_grid})
         *o += shift;
     L.cur += probe.cur;
     fill_meta(P, bcurves, tiles);

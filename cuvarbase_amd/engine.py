@@ -33,7 +33,7 @@ class Plan:
 
     @classmethod
     def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method,
-             grid_chunk_bytes=0, window=0, window_beta=0.0, point_capacity=0):
+             grid_chunk_bytes=0, window=0, window_beta=0.0, point_capacity=0, precision=0):
         _lib.init(0)
         n = len(curves)
         arr = (_lib.gls_curve * max(n, 1))()
@@ -44,7 +44,7 @@ class Plan:
                               input_kind=int(input_kind), method=int(method),
                               window=int(window), window_beta=float(window_beta),
                               grid_chunk_bytes=int(grid_chunk_bytes),
-                              point_capacity=int(point_capacity))
+                              point_capacity=int(point_capacity), precision=int(precision))
         h = C.c_void_p()
         _lib.call("gls_lomb_plan_create", C.byref(h), arr, C.c_int32(n), C.byref(prm))
         return cls(h.value)
@@ -173,12 +173,16 @@ class LombBatch:
     window, window_beta : spreading window of the NFFT path: the reference's
         Gaussian (``WINDOW_GAUSSIAN``) or the exponential of semicircle
         (``WINDOW_ES``, shape ``window_beta``; 0 picks the default)
+    precision : ``PRECISION_FP64_COORDS`` (FP64 time offsets, cells and
+        phases; the default) or ``PRECISION_FP32`` (the reference's FP32 time
+        arithmetic, for comparison)
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
                  input_kind="raw", ybar=None, yy=None, custom_freqs=False,
-                 grid_chunk_bytes=0, window=_lib.WINDOW_GAUSSIAN, window_beta=0.0):
+                 grid_chunk_bytes=0, window=_lib.WINDOW_GAUSSIAN, window_beta=0.0,
+                 precision=_lib.PRECISION_FP64_COORDS):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)
@@ -218,7 +222,7 @@ class LombBatch:
             method = _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT
             self._freqs = None
         self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method,
-                              grid_chunk_bytes, window, window_beta)
+                              grid_chunk_bytes, window, window_beta, precision=precision)
         # host staging of the inputs
         t = np.empty(obs, dtype=np.float64)
         if self.input_kind == _lib.INPUT_RAW:

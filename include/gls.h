@@ -100,7 +100,19 @@ typedef struct gls_params {
                            batch's own total: exactly that total.  Larger capacities let
                            gls_plan_rebind put other curves (more points) on the plan.
                            No reference counterpart. */
+    int32_t precision;   /* GLS_PRECISION_*; GLS_PRECISION_FP32 needs method NFFT */
+    int32_t pad;
 } gls_params;
+
+/* ---- precision of the time coordinates (gls_params.precision) ----------- */
+/* FP64 time offsets, grid cells and modulation phases, FP64 low-frequency
+ * fix-up (the default) */
+#define GLS_PRECISION_FP64_COORDS 0
+/* the reference's arithmetic for comparison: t rounded to FP32 after centring
+ * on its mean (cuvarbase/lombscargle.py:259), time offsets, grid coordinates
+ * and modulation phases in FP32 (cuvarbase/kernels/cunfft.cu:101-103,
+ * :233-241), no FP64 low-frequency fix-up */
+#define GLS_PRECISION_FP32        1
 
 /* One adjoint-NFFT item: sum_j y_j exp(2 pi i (f0 + k df) t_j), k < nf. */
 typedef struct gls_nfft_item {

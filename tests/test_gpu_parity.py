@@ -176,7 +176,7 @@ def test_long_curve_precision(config):
     f = freqs[0]
     b = engine.LombBatch(data, freqs, **CONFIGS[config]())
     P = b.run()[0].astype(np.float64)
-    Po = on.gls_nfft(t, y, dy, f, sigma=2.0, m=10)
+    Po = _long_oracle()
     err = np.max(np.abs(P - Po))
     kbest = int(np.argmax(Po))
     sel = np.concatenate([np.arange(kbest - 20, kbest + 21),
@@ -746,3 +746,49 @@ def test_drop_in_es_window_meets_target():
         assert int(np.argmax(ref)) == int(np.argmax(np.asarray(p)[sel]))
     print(f"drop-in ES sigma=1.35 m=5: max|dP| = {worst:.2e}")
     assert worst < TOL
+
+
+_LONG_ORACLE = {}
+
+
+def _long_oracle():
+    """FP64 oracle NFFT (Gaussian sigma = 2, m = 10) of the long curve, once."""
+    from cuvarbase_amd import synthetic
+
+    if not _LONG_ORACLE:
+        data, freqs = synthetic.long()
+        t, y, dy = data[0]
+        _LONG_ORACLE["P"] = on.gls_nfft(t, y, dy, freqs[0], sigma=2.0, m=10)
+    return _LONG_ORACLE["P"]
+
+
+def test_fp32_coordinates_precision_study():
+    """The precision study of the long curve (200,000 points over 3650 d,
+    fmax = 100 /d): FP32 everywhere (the reference's FP32 time, cell and
+    phase arithmetic) against FP32 with FP64 phase/cell reduction (the
+    default), both against the FP64 oracle.  On a 10-day baseline both
+    precisions meet 1e-4; over 3650 d only the FP64 reduction does."""
+    from cuvarbase_amd import _lib, engine, synthetic, tuning
+
+    cfg = tuning.choose_nfft_config(TOL)
+    sd, sf = synthetic.single()
+    g = np.load(os.path.join(GOLDEN, "single_direct.npz"))
+    for prec in (_lib.PRECISION_FP64_COORDS, _lib.PRECISION_FP32):
+        P = engine.LombBatch(sd, sf, precision=prec, **cfg).run()[0].astype(np.float64)
+        err = float(np.max(np.abs(P - g["power"])))
+        print(f"single workload precision={prec}: max|dP| = {err:.2e}")
+        assert err < TOL and np.argmax(P) == np.argmax(g["power"])
+
+    data, freqs = synthetic.long()
+    f = freqs[0]
+    Po = _long_oracle()
+    kbest = int(np.argmax(Po))
+    res = {}
+    for prec in (_lib.PRECISION_FP64_COORDS, _lib.PRECISION_FP32):
+        P = engine.LombBatch(data, freqs, precision=prec, **cfg).run()[0].astype(np.float64)
+        res[prec] = (float(np.max(np.abs(P - Po))), int(np.argmax(P)))
+        print(f"long curve precision={prec}: max|dP| = {res[prec][0]:.3e}, best period "
+              f"{1 / f[res[prec][1]]:.6f} d (oracle {1 / f[kbest]:.6f} d)")
+    assert res[_lib.PRECISION_FP64_COORDS][0] < TOL
+    assert res[_lib.PRECISION_FP64_COORDS][1] == kbest
+    assert res[_lib.PRECISION_FP32][0] > 10 * res[_lib.PRECISION_FP64_COORDS][0]
