@@ -253,6 +253,9 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--short-curves", type=int, default=100_000,
                     help="curves in the short-curve top-5 leg (0: skip)")
+    ap.add_argument("--spread", type=int, default=0,
+                    help="batched path's pass-1 accumulation: 0 FP32 atomics, 1 fixed point "
+                         "(bitwise reproducible); the server always runs fixed point")
     ap.add_argument("--grid-chunk-mib", type=int, default=0,
                     help="NFFT grid chunk size in MiB (0: no chunking)")
     ap.add_argument("--calib-copy-mib", type=int, default=2048,
@@ -286,7 +289,8 @@ def main():
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
 
     chunk = args.grid_chunk_mib << 20 if args.grid_chunk_mib > 0 else 0
-    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk, **win)
+    batch = engine.LombBatch(data, freqs, sigma=sigma, m=m, grid_chunk_bytes=chunk,
+                             spread=args.spread, **win)
     s = _device.Stream()
     batch.upload(stream=s)
     s.synchronize()
@@ -556,6 +560,7 @@ def main():
             "window_beta": cfg["window_beta"],
             "grid_cells_per_step": int(3 * sum(g[0] for g in geo)),
             "grid_chunk_bytes": int(chunk),
+            "spread": "fixed point" if args.spread else "fp32 atomics",
             "workspace_bytes": int(ws_bytes),
         },
         "roofline": {

@@ -37,6 +37,8 @@ WINDOW_ES = 1
 
 PRECISION_FP64_COORDS = 0
 PRECISION_FP32 = 1
+SPREAD_FLOAT = 0
+SPREAD_FIXED = 1
 
 STATIC_SCRATCH_BYTES = 128 << 20   # GLS_STATIC_SCRATCH_BYTES
 
@@ -69,7 +71,7 @@ class gls_params(C.Structure):
                 ("input_kind", C.c_int32), ("method", C.c_int32),
                 ("window", C.c_int32), ("window_beta", C.c_float),
                 ("grid_chunk_bytes", C.c_int64), ("point_capacity", C.c_int64),
-                ("precision", C.c_int32), ("pad", C.c_int32)]
+                ("precision", C.c_int32), ("spread", C.c_int32)]
 
 
 class gls_mh_curve(C.Structure):

@@ -85,6 +85,17 @@ __device__ __forceinline__ T block_max(T v, T* scratch) {
     return r;
 }
 
+// Exponent e of the fixed-point spreading scale 2^e of an LDS tile whose
+// cells can hold at most `bound` in magnitude: scaled cells stay below 2^30,
+// half the int32 range.  Clamped to the FP32 exponent range.
+__device__ __forceinline__ int spread_exponent(float bound) {
+    if (!(bound > 0.0f) || !isfinite(bound)) return 0;
+    int k;
+    (void)frexpf(bound, &k);   // bound < 2^k
+    const int e = 30 - k;
+    return e < -120 ? -120 : (e > 120 ? 120 : e);
+}
+
 // exp(2 pi i r) for a phase r given in revolutions as FP64: the integer part
 // is removed in FP64 (r - rint(r) in [-1/2, 1/2]) and only the reduced phase
 // is evaluated in FP32.

@@ -223,8 +223,8 @@ __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
 
 // Wide curves with a WideAux record: grid (wide curve, frequency, part);
 // each of the kLowSplit workgroups of a low frequency sums one slice of the
-// points, adds its six FP64 sums into the record, and the last to finish
-// forms the power.  Frequencies past the low run exit at once.
+// points and stores its six FP64 sums in the record; the last to finish adds
+// the slices in order and forms the power.  Frequencies past the low run exit at once.
 template <int OUT>
 __global__ __launch_bounds__(kLowWideNT) void lowfreq_split_kernel(
     const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
@@ -266,13 +266,16 @@ __global__ __launch_bounds__(kLowWideNT) void lowfreq_split_kernel(
     for (int w = 1; w < kLowWideNT / kWave; ++w)
         for (int q = 0; q < 6; ++q) acc[q] += red[w][q];
     WideAux& X = aux[blockIdx.x];
-    for (int q = 0; q < 6; ++q) atomicAdd(&X.low[k][q], acc[q]);
+    for (int q = 0; q < 6; ++q) X.low[k][blockIdx.z][q] = acc[q];
     __threadfence();
     if (atomicAdd(&X.low_done[k], 1) != kLowSplit - 1) return;
     __threadfence();
-    double sm[6];
-    for (int q = 0; q < 6; ++q)
-        sm[q] = __hip_atomic_load(&X.low[k][q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the slices in order, so the sum does not depend on which finished first
+    double sm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int z = 0; z < kLowSplit; ++z)
+        for (int q = 0; q < 6; ++q)
+            sm[q] += __hip_atomic_load(&X.low[k][z][q], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
     const double P = gls_power(sm[0], sm[1], sm[2], sm[3], sm[4], sm[5], st.yy, st.ybar,
                                pp.mode, pp.reg_c, pp.reg_s);
     if (OUT == 0) ((float*)out)[G.out + k] = (float)P;

@@ -114,6 +114,7 @@ struct PlanParams {
     int32_t window;    // GLS_WINDOW_*
     float es_beta;     // ES shape parameter
     int32_t fp32_coords;  // GLS_PRECISION_FP32: FP32 time offsets, cells and phases
+    int32_t spread_fixed; // GLS_SPREAD_FIXED: integer (fixed-point) LDS accumulation
     // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (the
     // library's resident device copy for (m, beta), es_table_device)
     const float* es_deconv;
@@ -150,9 +151,9 @@ constexpr int kMaxBuckets = 4096;   // n2 / c1 <= 2^12
 constexpr double kLowCycles = 3.0;
 constexpr int kLowMax = 256;
 constexpr int kLowSplit = 8;
-// (the record also holds, per low frequency, the six FP64 sums that the
-// kLowSplit workgroups of the low-frequency kernel add into, and how many of
-// them are done; the reductions kernel clears both at every execute)
+// (the record also holds, per low frequency, the six FP64 sums of each of the
+// kLowSplit workgroups of the low-frequency kernel, which the last of them adds
+// in slice order, and how many of them are done, cleared at every execute)
 struct WideAux {
     double ymean;
     double s0inv;
@@ -161,7 +162,7 @@ struct WideAux {
     int32_t yy_done;    // scatter blocks finished (variance partials)
     int32_t pad;
     int32_t hist[2][kMaxBuckets];
-    double low[kLowMax][6];
+    double low[kLowMax][kLowSplit][6];   // per low frequency and slice: the six sums
     int32_t low_done[kLowMax];
 };
 // Per-chunk partial reductions of a wide curve, one slot per chunk of

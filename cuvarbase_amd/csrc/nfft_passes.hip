@@ -31,7 +31,15 @@ using Ic = std::integral_constant<int, V>;
 // ---------------------------------------------------------------------------
 // pass 1: spread + column FFT + inter-pass twiddle
 // ---------------------------------------------------------------------------
-template <int N1>
+// FIX: fixed-point spreading (GLS_SPREAD_FIXED): the tile holds int32 pairs
+// while the taps are added, each tap rounded to a multiple of 2^-e, then is
+// converted to float2 in place before the FFT.  All the cells a point touches
+// in a tile lie in one row (its columns are consecutive and do not wrap), so
+// e is chosen per row: from the number of the tile's points in the row times
+// their largest |component| (window weights are at most 1), no cell of the
+// row can exceed 2^30.  Counts and maxima are integer LDS atomics, so e, the
+// roundings and the sums do not depend on the order of the points.
+template <int N1, bool FIX>
 __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict__ bgeom,
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
@@ -46,6 +54,7 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     __shared__ TwTable<N1> T;
     __shared__ float2 ehi[C1 * NH];   // exp(2 pi i u2 64 kh / n)
     __shared__ float2 elo[C1 * 64];   // exp(2 pi i u2 kl / n)
+    __shared__ signed char rexp[FIX ? N1 : 1];   // fixed-point exponent per row
     const int tid = threadIdx.x;
     const int n2 = 1 << p2;
     const int nb = n2 / C1;
@@ -120,7 +129,44 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     const float norm = G.g_norm;
     const float bl2 = es_beta * 1.4426950408889634f;          // beta log2(e)
     const int two_m = 2 * m;
+    // the tile row of a point's cells (the first in-tile tap is at column a,
+    // or, when the point's taps start inside the tile, at its first cell U)
+    auto row_of = [&](int U) {
+        const int k0 = (a - (U & (n2 - 1))) & (n2 - 1);
+        int cell = U + (k0 <= two_m ? k0 : 0);
+        cell = cell < 0 ? cell + n : (cell >= n ? cell - n : cell);
+        return cell >> p2;
+    };
+    if (FIX) {
+        // per row: point count (first N1 ints of the cleared tile) and the
+        // largest |component| as non-negative float bits (next N1 ints)
+        int* cnt = reinterpret_cast<int*>(tile);
+        int* mxb = cnt + N1;
+        auto count = [&](int U, float2 v) {
+            const int r = row_of(U);
+            atomicAdd(&cnt[r], 1);
+            atomicMax(&mxb[r], __float_as_int(fmaxf(fabsf(v.x), fabsf(v.y))));
+        };
+#pragma unroll
+        for (int f = 0; f < PF; ++f)
+            if (tid + f * NT1 < total) count(pU[f], pv[f]);
+        for (int q = tid + PF * NT1; q < total; q += NT1) {
+            const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
+            count(Ua[o], Va[o]);
+        }
+        __syncthreads();
+        for (int r = tid; r < N1; r += NT1)
+            rexp[r] = (signed char)spread_exponent((float)cnt[r] * __int_as_float(mxb[r]));
+        __syncthreads();
+        for (int i = tid; i < N1; i += NT1)   // the 2 N1 ints = N1 float2 used above
+            tile[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+    }
     auto spread = [&](int U, float fr, float2 v) {
+        if (FIX) {
+            const float sc = ldexpf(1.0f, rexp[row_of(U)]);
+            v = make_float2(v.x * sc, v.y * sc);
+        }
         const int col0 = U & (n2 - 1);
 #pragma unroll
         for (int jj = 0; jj < C1; ++jj) {
@@ -139,8 +185,14 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
                     wgt = exp2f(-d * d * binv) * norm;
                 }
                 float2* cellp = &tile[lds_at<BS>(u1, jj)];
-                atomicAdd(&cellp->x, v.x * wgt);
-                atomicAdd(&cellp->y, v.y * wgt);
+                if (FIX) {
+                    int* ip = reinterpret_cast<int*>(cellp);
+                    atomicAdd(ip, __float2int_rn(v.x * wgt));
+                    atomicAdd(ip + 1, __float2int_rn(v.y * wgt));
+                } else {
+                    atomicAdd(&cellp->x, v.x * wgt);
+                    atomicAdd(&cellp->y, v.y * wgt);
+                }
             }
         }
     };
@@ -152,6 +204,17 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
         spread(Ua[o], Fa[o], Va[o]);
     }
     __syncthreads();
+    if (FIX) {
+        // integer cells -> float2 with their row's scale (exact powers of two)
+        constexpr int LC = ilog2_c(C1);
+        for (int i = tid; i < N1 * C1; i += NT1) {
+            float2* cp = &tile[i + (i >> 4)];   // lds_at<BS>(i / C1, i % C1)
+            const int2 c = *reinterpret_cast<const int2*>(cp);
+            const float inv = ldexpf(1.0f, -(int)rexp[i >> LC]);
+            *cp = make_float2((float)c.x * inv, (float)c.y * inv);
+        }
+        __syncthreads();
+    }
 
     // column FFT; its last stage writes straight from registers to HBM with
     // the inter-pass twiddle exp(2 pi i u2 k1 / n) (consecutive lanes hold
@@ -327,9 +390,15 @@ hipError_t pass1_dispatch(const CurveGeom* bgeom, int ncurves, int ngrids, int p
     const int nb = (1 << p2) / C1;
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pass1_kernel<N1>), dim3((unsigned)nblocks), dim3(NT1), 0, s, bgeom,
-                       ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
-                       1.0f / ((float)pp.m + 0.5f));
+    const float ainv = 1.0f / ((float)pp.m + 0.5f);
+    if (pp.spread_fixed)
+        hipLaunchKernelGGL((pass1_kernel<N1, true>), dim3((unsigned)nblocks), dim3(NT1), 0, s,
+                           bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
+                           ainv);
+    else
+        hipLaunchKernelGGL((pass1_kernel<N1, false>), dim3((unsigned)nblocks), dim3(NT1), 0, s,
+                           bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
+                           ainv);
     return hipGetLastError();
 }
 

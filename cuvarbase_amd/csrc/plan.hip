@@ -608,6 +608,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         return fail(GLS_E_INVALID, "unknown precision");
     if (prm.precision == GLS_PRECISION_FP32 && prm.method != GLS_METHOD_NFFT)
         return fail(GLS_E_INVALID, "GLS_PRECISION_FP32 is an NFFT-method option");
+    if (prm.spread != GLS_SPREAD_FLOAT && prm.spread != GLS_SPREAD_FIXED)
+        return fail(GLS_E_INVALID, "unknown spread mode");
     gls_plan* P = new gls_plan();
     P->id = g_plan_ids.fetch_add(1);
     P->kind = 0;
@@ -620,6 +622,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.kind = 0;
     P->pp.window = (prm.method == GLS_METHOD_NFFT) ? prm.window : GLS_WINDOW_GAUSSIAN;
     P->pp.fp32_coords = prm.precision == GLS_PRECISION_FP32 ? 1 : 0;
+    P->pp.spread_fixed = prm.spread == GLS_SPREAD_FIXED ? 1 : 0;
     P->pp.es_beta = prm.window_beta > 0.f
                         ? prm.window_beta
                         : (float)(M_PI * (2 * prm.m + 1) * (1.0 - 0.5 / prm.sigma));

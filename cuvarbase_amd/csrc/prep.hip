@@ -353,7 +353,6 @@ __global__ __launch_bounds__(NT) void wide_clear_kernel(const CurveGeom* __restr
         X.hist[0][i] = 0;
         X.hist[1][i] = 0;
     }
-    for (int i = tid; i < kLowMax * 6; i += NT) (&X.low[0][0])[i] = 0.0;
     for (int i = tid; i < kLowMax; i += NT) X.low_done[i] = 0;
 }
 

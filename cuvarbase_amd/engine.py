@@ -33,7 +33,8 @@ class Plan:
 
     @classmethod
     def lomb(cls, curves, sigma, m, mode, reg_c, reg_s, input_kind, method,
-             grid_chunk_bytes=0, window=0, window_beta=0.0, point_capacity=0, precision=0):
+             grid_chunk_bytes=0, window=0, window_beta=0.0, point_capacity=0, precision=0,
+             spread=0):
         _lib.init(0)
         n = len(curves)
         arr = (_lib.gls_curve * max(n, 1))()
@@ -44,7 +45,8 @@ class Plan:
                               input_kind=int(input_kind), method=int(method),
                               window=int(window), window_beta=float(window_beta),
                               grid_chunk_bytes=int(grid_chunk_bytes),
-                              point_capacity=int(point_capacity), precision=int(precision))
+                              point_capacity=int(point_capacity), precision=int(precision),
+                              spread=int(spread))
         h = C.c_void_p()
         _lib.call("gls_lomb_plan_create", C.byref(h), arr, C.c_int32(n), C.byref(prm))
         return cls(h.value)
@@ -176,13 +178,16 @@ class LombBatch:
     precision : ``PRECISION_FP64_COORDS`` (FP64 time offsets, cells and
         phases; the default) or ``PRECISION_FP32`` (the reference's FP32 time
         arithmetic, for comparison)
+    spread : ``SPREAD_FLOAT`` (FP32 atomic accumulation, the default) or
+        ``SPREAD_FIXED`` (integer fixed-point accumulation: bitwise
+        reproducible results, about 18% more pass-1 time)
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
                  input_kind="raw", ybar=None, yy=None, custom_freqs=False,
                  grid_chunk_bytes=0, window=_lib.WINDOW_GAUSSIAN, window_beta=0.0,
-                 precision=_lib.PRECISION_FP64_COORDS):
+                 precision=_lib.PRECISION_FP64_COORDS, spread=_lib.SPREAD_FLOAT):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)
@@ -222,7 +227,8 @@ class LombBatch:
             method = _lib.METHOD_NFFT if use_fft else _lib.METHOD_DIRECT
             self._freqs = None
         self.plan = Plan.lomb(curves, sigma, m, mode, lam, lam, self.input_kind, method,
-                              grid_chunk_bytes, window, window_beta, precision=precision)
+                              grid_chunk_bytes, window, window_beta, precision=precision,
+                              spread=spread)
         # host staging of the inputs
         t = np.empty(obs, dtype=np.float64)
         if self.input_kind == _lib.INPUT_RAW:

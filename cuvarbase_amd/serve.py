@@ -14,9 +14,8 @@ next batch.  ``poll`` retires finished batches (event query, no blocking) and
 launches new ones.
 
 The per-curve kernel parameters (grid size, Gaussian width) depend only on the
-curve's own nf and on (sigma, m), never on the batch, so a request's result is
-the same alone and in a mix up to the order of floating-point additions in the
-LDS spreading.
+curve's own nf and on (sigma, m), never on the batch, and the spreading adds in
+fixed point, so a request's result is bitwise the same alone and in a mix.
 """
 from __future__ import annotations
 
@@ -260,6 +259,10 @@ class GLSServer:
         single larger request still runs, growing its slot's staging)
     sigma, m, window, window_beta : NFFT parameters (default: tuned for
         |dP| <= ``tol``)
+    spread : ``_lib.SPREAD_FIXED`` (the default) accumulates the spreading in
+        fixed point, so a request's result is bitwise the same alone and in any
+        mix; ``SPREAD_FLOAT`` uses FP32 atomics (equal to ~1e-7, the order of
+        FP32 additions)
     graph_batches : batches whose requests share one linear frequency grid
         (and have at most 16,384 points each) run as replays of a captured
         execute kept per (shape, slot) -- the plan is rebound to the new
@@ -276,7 +279,8 @@ class GLSServer:
     def __init__(self, hbm_budget=16e9, slots=3, max_batch_evals=3e8, max_batch_curves=8192,
                  sigma=None, m=None, tol=1e-4, mode=_lib.MODE_FLOATING_MEAN, zero_copy=False,
                  result_buffer_bytes=None, window=None, window_beta=0.0,
-                 max_batch_points=1 << 20, graph_batches=True, graphs_per_slot=2):
+                 max_batch_points=1 << 20, graph_batches=True, graphs_per_slot=2,
+                 spread=_lib.SPREAD_FIXED):
         _lib.init(0)
         cfg = choose_nfft_config(tol)
         if sigma is not None or m is not None or window is not None:
@@ -293,6 +297,7 @@ class GLSServer:
         self.max_batch_curves = int(max_batch_curves)
         self.max_batch_points = max(int(max_batch_points), 1)
         self.graph_batches = bool(graph_batches)
+        self.spread = int(spread)
         self.graphs_per_slot = max(int(graphs_per_slot), 1)
         self.graph_captures = 0
         self.graph_replays = 0
@@ -436,7 +441,7 @@ class GLSServer:
         cap = max(self.max_batch_points, obs)
         plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
                          _lib.METHOD_NFFT, window=self.window, window_beta=self.window_beta,
-                         point_capacity=cap)
+                         point_capacity=cap, spread=self.spread)
         while len(slot.graphs) >= self.graphs_per_slot:
             self._drop_graph(slot, next(iter(slot.graphs)))
         sizes = [24 * cap, 4 * nreq * nf, plan.workspace_bytes] + ([8 * nreq * k] if k else [])
@@ -480,7 +485,8 @@ class GLSServer:
         else:
             plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,
                              _lib.METHOD_DIRECT_CUSTOM if custom else _lib.METHOD_NFFT,
-                             window=self.window, window_beta=self.window_beta)
+                             window=self.window, window_beta=self.window_beta,
+                             spread=_lib.SPREAD_FIXED if custom else self.spread)
             sizes = [24 * max(obs, 1), 4 * max(out, 1), plan.workspace_bytes]
             sizes.append(8 * nreq * k if k else 0)
             if custom:

@@ -101,8 +101,24 @@ typedef struct gls_params {
                            gls_plan_rebind put other curves (more points) on the plan.
                            No reference counterpart. */
     int32_t precision;   /* GLS_PRECISION_*; GLS_PRECISION_FP32 needs method NFFT */
-    int32_t pad;
+    int32_t spread;      /* GLS_SPREAD_*: how pass 1 accumulates the window taps */
 } gls_params;
+
+/* ---- spreading accumulation (gls_params.spread) ------------------------- */
+/* FP32 atomic adds in LDS (the default): the order of additions depends on
+ * scheduling, so results can differ in the last bits from run to run and
+ * between a curve alone and in a batch (as the reference's global float
+ * atomics, cuvarbase/kernels/cunfft.cu:153-161) */
+#define GLS_SPREAD_FLOAT 0
+/* fixed point: every tap is rounded to an integer multiple of 2^-e and added
+ * with LDS integer atomics, which are exact and associative.  All the cells a
+ * point touches in a pass-1 tile lie in one grid row, so e is chosen per tile
+ * row from the number of points in the row times their largest |value|
+ * (no cell can exceed 2^30): accuracy as FP32 accumulation, and results
+ * bitwise the same for a curve alone, in any batch and on every run.  Costs a
+ * counting pass over each tile's points and a conversion of its cells: about
+ * 18% more pass-1 time on the 5,000-curve survey batch. */
+#define GLS_SPREAD_FIXED 1
 
 /* ---- precision of the time coordinates (gls_params.precision) ----------- */
 /* FP64 time offsets, grid cells and modulation phases, FP64 low-frequency

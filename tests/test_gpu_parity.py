@@ -133,11 +133,13 @@ def test_batch_equals_alone():
     assert worst < 1e-6
 
 
-def test_survey_sample_against_oracle():
-    from cuvarbase_amd import engine, synthetic
+@pytest.mark.parametrize("spread", ["fixed", "float"])
+def test_survey_sample_against_oracle(spread):
+    from cuvarbase_amd import _lib, engine, synthetic
 
     data, freqs = synthetic.survey(60, seed=22)
-    P = engine.LombBatch(data, freqs, sigma=2, m=8).run()
+    mode = _lib.SPREAD_FIXED if spread == "fixed" else _lib.SPREAD_FLOAT
+    P = engine.LombBatch(data, freqs, sigma=2, m=8, spread=mode).run()
     worst, same = 0.0, 0
     for i in range(0, 60, 6):
         t, y, dy = data[i]
@@ -146,7 +148,7 @@ def test_survey_sample_against_oracle():
         same += int(np.argmax(P[i]) == np.argmax(Po))
     for p in P:
         assert np.all(np.isfinite(p)) and np.all((p >= 0) & (p <= 1))
-    print(f"survey sample: max|dP| = {worst:.3e}, same best {same}/10")
+    print(f"survey sample ({spread} spreading): max|dP| = {worst:.3e}, same best {same}/10")
     assert worst < TOL and same == 10
 
 
@@ -571,7 +573,7 @@ def test_wide_curves_chunked_bucketing(config):
     tuned configuration the server runs), weighted input (drop-in) and the
     plain NFFT, in a batch mixed with short curves, against FP64 direct sums /
     DFT."""
-    from cuvarbase_amd import engine
+    from cuvarbase_amd import _lib, engine
     from cuvarbase_amd.cunfft import NFFTAsyncProcess
 
     rng = np.random.default_rng(5)
@@ -604,6 +606,10 @@ def test_wide_curves_chunked_bucketing(config):
     # a wide curve alone equals the same curve in the mixed batch
     alone = engine.LombBatch([data[2]], [freqs[2]], **cfg).run()[0]
     assert np.max(np.abs(alone - out[2])) < 2e-5
+    # with fixed-point spreading the wide curve alone is bitwise the batch's
+    fx = dict(cfg, spread=_lib.SPREAD_FIXED)
+    assert np.array_equal(engine.LombBatch([data[2]], [freqs[2]], **fx).run()[0],
+                          engine.LombBatch(data, freqs, **fx).run()[2])
 
     # plain NFFT of a wide curve against the direct DFT (scaled times, as in
     # the NFFT test on the reference's data above)

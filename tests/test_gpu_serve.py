@@ -1,10 +1,11 @@
 """Continuous-batching server on the GPU.
 
-A request's periodogram served in a mix and computed alone differ only by the
-order of FP32 additions in the LDS spreading, which the deconvolution
-amplifies at the band edges (up to ~100x for the Gaussian at sigma = 1.45,
-m = 8; less for the exponential-of-semicircle window), so the two agree to
-MIX_TOL = 2e-5, well inside the 1e-4 parity bar against FP64.
+The server spreads in fixed point by default, so a request's periodogram is
+bitwise the same served in a mix and computed alone with the same setting
+(integer accumulation is exact and order-free).  MIX_TOL = 2e-5 is the bar
+where FP32-atomic spreading (GLS_SPREAD_FLOAT, the batched engine's default)
+is involved: the order of its additions changes with the batch and the
+deconvolution amplifies that round-off at the band edges.
 """
 import numpy as np
 import pytest
@@ -32,11 +33,11 @@ def test_each_result_same_alone_and_in_the_mix():
     for r, d, f in zip(st["requests_obj"], data, freqs):
         assert r.done and r.latency > 0
         alone = engine.LombBatch([d], [f], sigma=srv.sigma, m=srv.m, window=srv.window,
-                                 window_beta=srv.window_beta).run()[0]
+                                 window_beta=srv.window_beta, spread=srv.spread).run()[0]
         worst = max(worst, float(np.max(np.abs(r.power.astype(float) - alone))))
+        assert np.array_equal(r.power, alone)
     print(f"served {st['requests']} requests in {st['batches']} batches, p50 {st['p50_ms']:.1f} "
           f"ms, p99 {st['p99_ms']:.1f} ms; alone vs mix max|dP| = {worst:.2e}")
-    assert worst < MIX_TOL
 
 
 def test_admission_waits_for_memory():
@@ -170,7 +171,8 @@ def test_graph_served_top5_matches_eager_and_oracle():
     assert sg.graph_captures == 1 and sg.graph_replays == 3
     assert se.graph_captures == 0 and se.graph_replays == 0
     for i, (a, b, d) in enumerate(zip(rg, re_, data)):
-        np.testing.assert_allclose(a.top_power, b.top_power, atol=MIX_TOL)
+        np.testing.assert_array_equal(a.top_power, b.top_power)
+        np.testing.assert_array_equal(a.top_index, b.top_index)
         Po = og.gls_direct(*d, grid)
         order = np.argsort(-Po, kind="stable")[:5]
         for q in range(5):
@@ -180,3 +182,52 @@ def test_graph_served_top5_matches_eager_and_oracle():
             jb = int(b.top_index[q])
             assert j == jb or abs(Po[j] - Po[jb]) <= 1e-4
         assert np.all((a.fap >= 0) & (a.fap <= 1))
+
+
+def test_fixed_point_spreading_is_bitwise_reproducible():
+    """GLS_SPREAD_FIXED: every request's periodogram served in a mix is
+    bitwise equal to the same curve computed alone (and to a second run),
+    curves above 16,384 points included, and meets 1e-4 against FP64."""
+    from oracle import gls as og
+
+    from cuvarbase_amd import _lib, engine
+    from cuvarbase_amd.serve import GLSServer
+
+    arrivals, data, freqs = _stream()
+    assert sum(len(d[0]) > 16384 for d in data) >= 3
+    srv = GLSServer(hbm_budget=8e9, slots=3, max_batch_evals=2e7, spread=_lib.SPREAD_FIXED)
+    st = srv.serve_stream(arrivals, data, freqs)
+    assert st["batches"] > 1
+    cfg = dict(sigma=srv.sigma, m=srv.m, window=srv.window, window_beta=srv.window_beta)
+    rng = np.random.default_rng(5)
+    again = engine.LombBatch(data, freqs, spread=_lib.SPREAD_FIXED, **cfg).run()
+    worst = 0.0
+    for i, (r, d, f) in enumerate(zip(st["requests_obj"], data, freqs)):
+        alone = engine.LombBatch([d], [f], spread=_lib.SPREAD_FIXED, **cfg).run()[0]
+        assert np.array_equal(r.power, alone), i
+        assert np.array_equal(again[i], alone), i
+        if i % 4 == 0:
+            sel = np.unique(np.concatenate([np.arange(20), [int(np.argmax(alone))],
+                                            rng.choice(len(f), 40, replace=False)]))
+            worst = max(worst, float(np.max(np.abs(alone[sel] - og.gls_direct(*d, f[sel])))))
+    print(f"fixed-point spreading: served == alone bitwise for {len(data)} requests; "
+          f"max|dP| vs FP64 {worst:.2e}")
+    assert worst < 1e-4
+
+
+def test_float_spreading_alone_vs_mix_within_tolerance():
+    """GLS_SPREAD_FLOAT through the server: results of the mix equal the
+    curve alone within MIX_TOL (FP32 atomic order)."""
+    from cuvarbase_amd import _lib, engine
+    from cuvarbase_amd.serve import GLSServer
+
+    arrivals, data, freqs = _stream(n=24, seed=43)
+    srv = GLSServer(hbm_budget=8e9, slots=3, max_batch_evals=2e7, spread=_lib.SPREAD_FLOAT)
+    st = srv.serve_stream(arrivals, data, freqs)
+    worst = 0.0
+    for r, d, f in zip(st["requests_obj"], data, freqs):
+        alone = engine.LombBatch([d], [f], sigma=srv.sigma, m=srv.m, window=srv.window,
+                                 window_beta=srv.window_beta, spread=_lib.SPREAD_FLOAT).run()[0]
+        worst = max(worst, float(np.max(np.abs(r.power.astype(float) - alone))))
+    print(f"float spreading: alone vs mix max|dP| = {worst:.2e}")
+    assert worst < MIX_TOL

@@ -3,10 +3,10 @@
 Both run the same plan on the same device inputs; the static entry point uses
 the library's static device scratch instead of a caller workspace.  Two calls
 in a row through the static entry point, the second with a smaller batch, each
-compared with the workspace entry point: |dP| <= 2e-5 (the spreading adds in
-LDS with float atomics, so two executes of one plan may differ by round-off)
-and the same best index per curve; both also against the FP64 oracle at
-|dP| <= 1e-4.
+compared with the workspace entry point: |dP| <= 2e-5 (the default spreading
+adds in LDS with float atomics, so two executes of one plan may differ by
+round-off) and the same best index per curve; both also against the FP64
+oracle at |dP| <= 1e-4.
 """
 import numpy as np
 import pytest
