@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--short-curves", type=int, default=100_000,
                     help="curves in the short-curve top-5 leg (0: skip)")
+    ap.add_argument("--e2e", type=int, default=1,
+                    help="time the end-to-end (host memory to host memory) batch (0: skip)")
     ap.add_argument("--spread", type=int, default=0,
                     help="batched path's pass-1 accumulation: 0 FP32 atomics, 1 fixed point "
                          "(bitwise reproducible); the server always runs fixed point")
@@ -367,7 +369,7 @@ def main():
     # end to end: the same batch from page-locked host memory to page-locked
     # host memory (H2D of t, y, dy + execute + D2H of every power, or of the
     # top-5 peaks per curve), on the execute stream
-    e2e = end_to_end(batch, s, reps=max(1, min(args.steps, 3)))
+    e2e = end_to_end(batch, s, reps=max(1, min(args.steps, 3))) if args.e2e else None
 
     # plain path: the drop-in API, one curve per run() + finish(), at the same
     # NFFT configuration as the batched path
