@@ -337,16 +337,18 @@ def main():
     ncurves = batch.ncurves
     ws_bytes = batch.plan.workspace_bytes
     geo = [batch.plan.geometry(i) for i in range(ncurves)]
+    tiles = [batch.plan.tiles(i) for i in range(ncurves)]
 
-    # dominant kernel: the kernel template (pass-1 column length n1 or pass-2
-    # row length n2) with the most time over the timed steps
-    split = {g[0]: (g[1], g[2]) for g in geo}
+    # dominant kernel: the kernel template (pass-1 column length n1 and tile
+    # c1, or pass-2 row length n2 and tile cr) with the most time over the
+    # timed steps
+    split = {g[0]: (g[1], g[2], tl[0], tl[1]) for g, tl in zip(geo, tiles)}
     by_kernel, by_kind = {}, {}
     for (name, p), (c, ms, by) in stage_acc.items():
         if name == "pass1":
-            key = f"pass1_kernel<{split[p][0]}>"
+            key = f"pass1_kernel<{split[p][0]}, {split[p][2]}, {'true' if args.spread else 'false'}>"
         elif name == "pass2":
-            key = f"pass2_kernel<{int(math.log2(split[p][1]))}, 0>"
+            key = f"pass2_kernel<{int(math.log2(split[p][1]))}, {split[p][3]}, 0>"
         else:
             key = name
         for d, k in ((by_kernel, key), (by_kind, name)):

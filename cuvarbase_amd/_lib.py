@@ -120,6 +120,11 @@ _PROTOS = {
     "gls_plan_grid_size": [_vp, C.c_int32, C.POINTER(C.c_int64)],
     "gls_plan_curve_geometry": [_vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double)],
+    "gls_plan_curve_tiles": [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "gls_tile_set": [C.c_int32, C.c_int32, C.c_int32, C.c_int32],
+    "gls_tile_get": [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "gls_grid_cost_set": [C.c_int32, C.c_int32, C.c_double],
+    "gls_grid_cost_get": [C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)],
     "gls_lomb_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gls_lomb_execute_static": [_vp, _vp, _vp, _vp, _vp, _vp],

@@ -9,6 +9,7 @@
 
 #include "../../include/gls.h"
 #include "sizes.hpp"
+#include "tiles.hpp"
 
 namespace gls {
 
@@ -122,21 +123,14 @@ struct PlanParams {
 
 constexpr int kEsTab = 16384;   // ES deconvolution table intervals over xi in [0, 1/2]
 
-// pass-1 columns per tile: power of two with n1 * c1 <= 4096, at most 16
-constexpr int pass1_cols(int n1) {
-    int c = 1;
-    while (c < 16 && n1 * c * 2 <= 4096) c *= 2;
-    return c;
-}
-// pass-2 rows per tile (power of two): tiles of n2 * cr = 1024 row outputs for
-// rows of up to 256 (more, smaller blocks in flight), 2048 for longer rows
-// (fewer, wider row segments per read); measured on the survey batch
-constexpr int pass2_rows(int p2) {
-    int c = (p2 <= 8 ? 1024 : 2048) >> p2;
-    if (c > 16) c = 16;
-    if (c < 1) c = 1;
-    return c;
-}
+// tiles of a size bucket (n1, p2): the caller's override (gls_tile_set), else
+// the measured table compiled in, else the default rules (tiles.hpp)
+struct BucketTiles {
+    int c1, cr;
+};
+// whether this library has a kernel for the tile (tiles_gen.hpp lists them)
+bool pass1_tile_built(int n1, int c1);
+bool pass2_tile_built(int p2, int cr);
 
 // launchers (defined in the kernel translation units)
 // curves of more than kWidePrep points are prepared by 1024-thread blocks
@@ -187,14 +181,15 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        hipStream_t s);
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
-hipError_t launch_pass1(int n1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
-                        PointArrays pa, const int32_t* bkt, float2* grids,
+// (c1: the bucket's pass-1 tile, cr: its pass-2 tile)
+hipError_t launch_pass1(int n1, int c1, const CurveGeom* bgeom, int ncurves, int ngrids,
+                        int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s);
-hipError_t launch_pass2_lomb(int p2, int n1, const CurveGeom* bgeom,
+hipError_t launch_pass2_lomb(int p2, int cr, int n1, const CurveGeom* bgeom,
                              const CurveState* state, int ncurves, const float2* grids,
                              const PlanParams& pp, void* power, int out_f64,
                              hipStream_t s);
-hipError_t launch_pass2_nfft(int p2, int n1, const CurveGeom* bgeom,
+hipError_t launch_pass2_nfft(int p2, int cr, int n1, const CurveGeom* bgeom,
                              const CurveState* state, int ncurves, const float2* grids,
                              const PlanParams& pp, float2* out, hipStream_t s);
 hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,

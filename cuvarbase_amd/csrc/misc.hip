@@ -100,15 +100,19 @@ __global__ __launch_bounds__(NT) void topk_kernel(const CurveGeom* __restrict__ 
 // Top-k for k <= KM in one read of the powers: each thread keeps a sorted
 // list of its KM best (value desc, index asc) over a strided slice, then k
 // rounds of block argmax over the lists' heads pick the winners in order.
-// Same order and ties as topk_kernel.
+// Same order and ties as topk_kernel (the order is total, so the slices and
+// the order in which a thread visits its elements do not matter).  The slice
+// is read UNR elements at a time, all loads issued before the first compare,
+// so that one block keeps enough reads in flight for a long curve.
+constexpr int NTK = 512;
 template <int KM>
-__global__ __launch_bounds__(NT) void topk_small_kernel(const CurveGeom* __restrict__ geom,
+__global__ __launch_bounds__(NTK) void topk_small_kernel(const CurveGeom* __restrict__ geom,
                                                         const float* __restrict__ power,
                                                         const uint8_t* __restrict__ mask,
                                                         int K, float* __restrict__ top_power,
                                                         int32_t* __restrict__ top_index) {
-    __shared__ float sv[NT / kWave];
-    __shared__ int si[NT / kWave];
+    __shared__ float sv[NTK / kWave];
+    __shared__ int si[NTK / kWave];
     __shared__ int win;
     const int c = blockIdx.x;
     const CurveGeom& G = geom[c];
@@ -123,11 +127,9 @@ __global__ __launch_bounds__(NT) void topk_small_kernel(const CurveGeom* __restr
         lv[q] = -INFINITY;
         li[q] = 0x7fffffff;
     }
-    for (int k = tid; k < G.nf; k += NT) {
-        if (mask && mask[G.out + k]) continue;
-        const float v = power[G.out + k];
-        if (!(v >= 0.f)) continue;   // -1 sentinel and NaN
-        if (!better(v, k, lv[KM - 1], li[KM - 1])) continue;
+    auto offer = [&](float v, int k) {
+        if (!(v >= 0.f)) return;   // -1 sentinel, NaN and masked
+        if (!better(v, k, lv[KM - 1], li[KM - 1])) return;
         // insert into the sorted list (registers: unrolled shift)
         float cv = v;
         int ci = k;
@@ -142,7 +144,25 @@ __global__ __launch_bounds__(NT) void topk_small_kernel(const CurveGeom* __restr
                 ci = ti;
             }
         }
+    };
+    constexpr int UNR = 4;
+    const float* const pw = power + G.out;
+    const uint8_t* const mk = mask ? mask + G.out : nullptr;
+    const int nf = G.nf;
+    int k = tid;
+    for (; k + (UNR - 1) * NTK < nf; k += UNR * NTK) {
+        float v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) v[u] = pw[k + u * NTK];
+        if (mk) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+                if (mk[k + u * NTK]) v[u] = -1.f;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) offer(v[u], k + u * NTK);
     }
+    for (; k < nf; k += NTK) offer((mk && mk[k]) ? -1.f : pw[k], k);
     int head = 0;   // next unpicked entry of this thread's list
     for (int round = 0; round < K; ++round) {
         float best = -INFINITY;
@@ -171,7 +191,7 @@ __global__ __launch_bounds__(NT) void topk_small_kernel(const CurveGeom* __restr
         if (tid == 0) {
             float bv = sv[0];
             int bx = si[0];
-            for (int q = 1; q < NT / kWave; ++q)
+            for (int q = 1; q < NTK / kWave; ++q)
                 if (better(sv[q], si[q], bv, bx)) {
                     bv = sv[q];
                     bx = si[q];
@@ -207,7 +227,7 @@ hipError_t launch_topk(const CurveGeom* d_geom, int ncurves, const float* power,
                        hipStream_t s) {
     if (ncurves <= 0 || k <= 0) return hipSuccess;
     if (k <= 8)
-        hipLaunchKernelGGL((topk_small_kernel<8>), dim3(ncurves), dim3(NT), 0, s, d_geom, power,
+        hipLaunchKernelGGL((topk_small_kernel<8>), dim3(ncurves), dim3(NTK), 0, s, d_geom, power,
                            mask, k, top_power, top_index);
     else
         hipLaunchKernelGGL(topk_kernel, dim3(ncurves), dim3(NT), 0, s, d_geom, power, mask, k,

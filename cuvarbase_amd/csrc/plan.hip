@@ -32,7 +32,7 @@ struct gls_plan {
     int ncurves = 0;
     std::vector<gls::CurveGeom> geom;
     struct Bucket {
-        int n1 = 0, p2 = 0, c1 = 0;
+        int n1 = 0, p2 = 0, c1 = 0, cr = 0;   // size and tiles (columns, rows)
         int64_t first = 0;  // offset (curves) into the bucket curve list
         int count = 0;
     };
@@ -110,8 +110,85 @@ double gauss_b(double sigma, int m) {
     return 2.0 * sigma * (double)m / ((2.0 * sigma - 1.0) * M_PI);
 }
 
-// smallest-cost grid n = n1 * 2^p2 >= need from the size catalogue
-bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
+// ---- tiles and measured costs of the size buckets ------------------------------
+std::mutex g_tile_mu;
+std::map<std::pair<int, int>, BucketTiles> g_tile_override;   // gls_tile_set
+std::map<std::pair<int, int>, double> g_cost_override;        // gls_grid_cost_set
+
+// the measured table compiled in (tiles_gen.hpp), else the default rules
+BucketTiles table_tiles(int n1, int p2) {
+    BucketTiles t{default_pass1_cols(n1), default_pass2_rows(p2)};
+#define GLS_TILE_ROW(N, P, C, R, PS) \
+    if (n1 == N && p2 == P) t = BucketTiles{C, R};
+    GLS_TILE_TABLE(GLS_TILE_ROW)
+#undef GLS_TILE_ROW
+    return t;
+}
+
+// measured time of the two passes per grid cell (picoseconds) of bucket
+// (n1, p2) at its tiles, from the table compiled in; < 0 if not measured
+double table_cost(int n1, int p2) {
+#define GLS_COST_ROW(N, P, C, R, PS) \
+    if (n1 == N && p2 == P) return PS;
+    GLS_TILE_TABLE(GLS_COST_ROW)
+#undef GLS_COST_ROW
+    return -1.0;
+}
+
+// model cost per cell of an unmeasured bucket: one pass over the cells plus
+// 0.12 per LDS radix stage of the two FFTs, in the table's units
+double model_cost(int n1, int p2) {
+    return GLS_MODEL_PS_PER_UNIT * (1.0 + 0.12 * (radix_stages(n1) + radix_stages(1 << p2)));
+}
+
+// cost per cell of every bucket, looked up once per plan: the caller's
+// override (gls_grid_cost_set), else the measured table, else the model
+struct GridCosts {
+    double ps[kMaxLog2Row - kMinLog2Row + 1][kNumPass1Sizes];
+    GridCosts() {
+        std::map<std::pair<int, int>, double> ov;
+        {
+            std::lock_guard<std::mutex> lk(g_tile_mu);
+            ov = g_cost_override;
+        }
+        for (int p2 = kMinLog2Row; p2 <= kMaxLog2Row; ++p2)
+            for (int i = 0; i < kNumPass1Sizes; ++i) {
+                const int n1 = kPass1Sizes[i];
+                auto it = ov.find({n1, p2});
+                double c = it != ov.end() ? it->second : table_cost(n1, p2);
+                if (c < 0) c = model_cost(n1, p2);
+                ps[p2 - kMinLog2Row][i] = c;
+            }
+    }
+};
+
+// the tiles plans made now use for bucket (n1, p2)
+BucketTiles current_tiles(int n1, int p2) {
+    BucketTiles t = table_tiles(n1, p2);
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    auto it = g_tile_override.find({n1, p2});
+    if (it != g_tile_override.end()) {
+        if (it->second.c1) t.c1 = it->second.c1;
+        if (it->second.cr) t.cr = it->second.cr;
+    }
+    return t;
+}
+
+// the tiles of one plan's buckets, looked up once per bucket
+struct TileCache {
+    std::map<std::pair<int, int>, BucketTiles> seen;
+    BucketTiles get(int n1, int p2) {
+        auto it = seen.find({n1, p2});
+        if (it != seen.end()) return it->second;
+        return seen[{n1, p2}] = current_tiles(n1, p2);
+    }
+};
+
+// cheapest grid n = n1 * 2^p2 >= need from the size catalogue, by cost per
+// cell times cells (a point's 2m + 1 taps and a pass-1 tile must fit in one
+// row: 2m + c1 <= n2)
+bool choose_grid(int64_t need, int m, TileCache& tiles, const GridCosts& costs, int* n1_out,
+                 int* p2_out) {
     double best = 0.0;
     bool found = false;
     if (need < kMinGrid) need = kMinGrid;
@@ -121,9 +198,8 @@ bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
             const int n1 = kPass1Sizes[i];
             const int64_t n = (int64_t)n1 * n2;
             if (n < need || n > kMaxGrid) continue;
-            if (2 * m + pass1_cols(n1) > n2) continue;
-            const double cost =
-                (double)n * (1.0 + 0.12 * (radix_stages(n1) + radix_stages((int)n2)));
+            if (2 * m + 16 > n2 && 2 * m + tiles.get(n1, p2).c1 > n2) continue;
+            const double cost = (double)n * costs.ps[p2 - kMinLog2Row][i];
             if (!found || cost < best) {
                 best = cost;
                 *n1_out = n1;
@@ -143,17 +219,19 @@ bool choose_grid(int64_t need, int m, int* n1_out, int* p2_out) {
 int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
                        std::vector<int32_t>& bcurves) {
     const int m = P->pp.m;
+    TileCache tiles;
+    const GridCosts costs;
     std::map<std::pair<int, int>, std::vector<int32_t>> by_size;
     int64_t bkt_ints = 0;
     for (int i = 0; i < P->ncurves; ++i) {
         CurveGeom& G = P->geom[i];
         const int64_t need = (int64_t)std::ceil(P->sigma * (double)G.nf);
         int n1 = 0, p2 = 0;
-        if (!choose_grid(need, m, &n1, &p2))
+        if (!choose_grid(need, m, tiles, costs, &n1, &p2))
             return fail(GLS_E_UNSUPPORTED,
                         "no grid for curve " + std::to_string(i) + " (nf=" +
                             std::to_string(G.nf) + ", sigma*nf must be <= 2^24)");
-        const int c1 = pass1_cols(n1);
+        const int c1 = tiles.get(n1, p2).c1;
         const int n2 = 1 << p2;
         const int64_t n = (int64_t)n1 * n2;
         G.n = (int32_t)n;
@@ -176,7 +254,9 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         gls_plan::Bucket B;
         B.n1 = kv.first.first;
         B.p2 = kv.first.second;
-        B.c1 = pass1_cols(B.n1);
+        const BucketTiles bt = tiles.get(B.n1, B.p2);
+        B.c1 = bt.c1;
+        B.cr = bt.cr;
         B.first = (int64_t)bcurves.size();
         B.count = 0;
         int64_t acc = 0;   // grid elements of the open entry
@@ -509,14 +589,15 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             const auto& B = P->buckets[bi];
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
             const CurveGeom* bgeom = d_bgeom + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, bgeom, B.count, ngrids, B.p2, pa, bkt, grids, pp, ls));
+            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, bgeom, B.count, ngrids, B.p2, pa, bkt, grids,
+                                     pp, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
-                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.n1, bgeom, state, B.count, grids, pp,
+                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
                                               (float2*)out, ls));
             else
-                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.n1, bgeom, state, B.count, grids, pp, out,
-                                              out_f64, ls));
+                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
+                                              out, out_f64, ls));
             GLS_HIP_TRY(mark());
         }
         if (fork) {
@@ -804,6 +885,83 @@ int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n, int32_t
     if (n1) *n1 = nfft ? G.n1 : 0;
     if (n2) *n2 = nfft ? (1 << G.p2) : 0;
     if (b) *b = nfft ? G.b : 0.0;
+    return GLS_OK;
+}
+
+int gls_plan_curve_tiles(const gls_plan* plan, int32_t i, int32_t* c1, int32_t* cr) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (i < 0 || i >= plan->ncurves) return fail(GLS_E_INVALID, "curve index out of range");
+    const CurveGeom& G = plan->geom[i];
+    int tc = 0, tr = 0;
+    if (plan->pp.method == GLS_METHOD_NFFT)
+        for (const auto& B : plan->buckets)
+            if (B.n1 == G.n1 && B.p2 == G.p2) {
+                tc = B.c1;
+                tr = B.cr;
+                break;
+            }
+    if (c1) *c1 = tc;
+    if (cr) *cr = tr;
+    return GLS_OK;
+}
+
+int gls_tile_set(int32_t n1, int32_t p2, int32_t c1, int32_t cr) {
+    bool in_catalogue = false;
+    for (int v : kPass1Sizes) in_catalogue |= v == n1;
+    if (!in_catalogue || p2 < kMinLog2Row || p2 > kMaxLog2Row)
+        return fail(GLS_E_INVALID, "(n1, p2) is not a grid size of the catalogue");
+    if (c1 < 0 || cr < 0) return fail(GLS_E_INVALID, "tiles must be >= 0");
+    if (c1 && !(pass1_tile_ok(n1, c1) && pass1_tile_built(n1, c1)))
+        return fail(GLS_E_UNSUPPORTED, "no pass-1 kernel with " + std::to_string(c1) +
+                                           " columns of " + std::to_string(n1) +
+                                           " cells in this library");
+    if (cr && !(pass2_tile_ok(p2, cr) && pass2_tile_built(p2, cr)))
+        return fail(GLS_E_UNSUPPORTED, "no pass-2 kernel with " + std::to_string(cr) +
+                                           " rows of 2^" + std::to_string(p2) +
+                                           " cells in this library");
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    if (!c1 && !cr) g_tile_override.erase({n1, p2});
+    else g_tile_override[{n1, p2}] = BucketTiles{c1, cr};
+    return GLS_OK;
+}
+
+int gls_tile_get(int32_t n1, int32_t p2, int32_t* c1, int32_t* cr) {
+    const BucketTiles t = current_tiles(n1, p2);
+    if (c1) *c1 = t.c1;
+    if (cr) *cr = t.cr;
+    return GLS_OK;
+}
+
+int gls_grid_cost_set(int32_t n1, int32_t p2, double ps_per_cell) {
+    bool in_catalogue = false;
+    for (int v : kPass1Sizes) in_catalogue |= v == n1;
+    if (!in_catalogue || p2 < kMinLog2Row || p2 > kMaxLog2Row)
+        return fail(GLS_E_INVALID, "(n1, p2) is not a grid size of the catalogue");
+    if (!std::isfinite(ps_per_cell)) return fail(GLS_E_INVALID, "cost must be finite");
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    if (ps_per_cell <= 0) g_cost_override.erase({n1, p2});
+    else g_cost_override[{n1, p2}] = ps_per_cell;
+    return GLS_OK;
+}
+
+int gls_grid_cost_get(int32_t n1, int32_t p2, double* ps_per_cell, int32_t* source) {
+    double c = -1.0;
+    int src = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_cost_override.find({n1, p2});
+        if (it != g_cost_override.end()) {
+            c = it->second;
+            src = 2;
+        }
+    }
+    if (src == 0) {
+        c = table_cost(n1, p2);
+        src = c >= 0 ? 1 : 0;
+        if (c < 0) c = model_cost(n1, p2);
+    }
+    if (ps_per_cell) *ps_per_cell = c;
+    if (source) *source = src;
     return GLS_OK;
 }
 

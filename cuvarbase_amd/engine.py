@@ -25,6 +25,36 @@ def linear_grid(freqs):
     return float(freqs[0]), float(freqs[1] - freqs[0]), int(len(freqs))
 
 
+def set_tiles(n1, p2, c1=0, cr=0):
+    """Override the NFFT pass tiles of size bucket (n1, p2) for plans made
+    from now on (0 keeps the library's choice for that pass; both 0 remove
+    the override).  Raises GLSError if the library has no such kernel."""
+    _lib.call("gls_tile_set", C.c_int32(n1), C.c_int32(p2), C.c_int32(c1), C.c_int32(cr))
+
+
+def get_tiles(n1, p2):
+    """(c1, cr) that a plan made now would use for size bucket (n1, p2)."""
+    c1 = C.c_int32(0)
+    cr = C.c_int32(0)
+    _lib.call("gls_tile_get", C.c_int32(n1), C.c_int32(p2), C.byref(c1), C.byref(cr))
+    return c1.value, cr.value
+
+
+def set_grid_cost(n1, p2, ps_per_cell):
+    """Override the cost per grid cell (picoseconds) by which plans made from
+    now on choose grid sizes, for bucket (n1, p2); <= 0 removes it."""
+    _lib.call("gls_grid_cost_set", C.c_int32(n1), C.c_int32(p2), C.c_double(ps_per_cell))
+
+
+def get_grid_cost(n1, p2):
+    """(cost per cell in picoseconds, source) of bucket (n1, p2); source 0 is
+    the model, 1 the measured table, 2 an override."""
+    c = C.c_double(0)
+    src = C.c_int32(0)
+    _lib.call("gls_grid_cost_get", C.c_int32(n1), C.c_int32(p2), C.byref(c), C.byref(src))
+    return c.value, src.value
+
+
 class Plan:
     """Owns a ``gls_plan`` handle."""
 
@@ -98,6 +128,14 @@ class Plan:
         _lib.call("gls_plan_curve_geometry", self.handle, C.c_int32(i), C.byref(n), C.byref(n1),
                   C.byref(n2), C.byref(b))
         return n.value, n1.value, n2.value, b.value
+
+    def tiles(self, i):
+        """(c1, cr) of curve i's size bucket: grid columns per pass-1
+        workgroup, grid rows per pass-2 workgroup."""
+        c1 = C.c_int32(0)
+        cr = C.c_int32(0)
+        _lib.call("gls_plan_curve_tiles", self.handle, C.c_int32(i), C.byref(c1), C.byref(cr))
+        return c1.value, cr.value
 
     def set_fork(self, enable=True):
         """Run independent size buckets concurrently on the library's streams."""

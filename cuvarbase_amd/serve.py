@@ -266,9 +266,13 @@ class GLSServer:
     graph_batches : batches whose requests share one linear frequency grid
         (and have at most 16,384 points each) run as replays of a captured
         execute kept per (shape, slot) -- the plan is rebound to the new
-        curves and only its descriptors and the inputs are uploaded; at most
-        ``graphs_per_slot`` shapes are kept per slot (least recently used
-        first out), their device blocks held in the arena meanwhile
+        curves and only its descriptors and the inputs are uploaded.  A shape
+        is captured the second time a batch of it runs (the first runs
+        eagerly: a capture costs more host time than an eager launch, and
+        shapes seen once -- a lone request on its own grid -- would never
+        repay it); at most ``graphs_per_slot`` shapes are kept per slot
+        (least recently used first out), their device blocks held in the
+        arena meanwhile
     zero_copy : results are views into a page-locked result arena of
         ``result_buffer_bytes`` (default 4 B x max_batch_evals x (slots + 1))
         that the D2H copies write directly; each request's memory returns to
@@ -301,6 +305,7 @@ class GLSServer:
         self.graphs_per_slot = max(int(graphs_per_slot), 1)
         self.graph_captures = 0
         self.graph_replays = 0
+        self._shapes_seen = OrderedDict()   # batch shapes run at least once (bounded)
         self.arena = DeviceArena(hbm_budget)
         self.zero_copy = bool(zero_copy)
         self.host_arena = None
@@ -336,6 +341,7 @@ class GLSServer:
         self.batches_launched = 0
         self.evals_done = 0
         self.graph_captures = self.graph_replays = 0
+        self._shapes_seen.clear()
 
     # -- client side ---------------------------------------------------------
     def submit(self, t, y, dy, freqs, arrival=None, custom_freqs=False, k=0, mode=None,
@@ -425,10 +431,13 @@ class GLSServer:
         for blk in entry.blocks:
             self.arena.release(blk)
 
+    SHAPES_REMEMBERED = 1024
+
     def _graph_entry(self, slot, key, curves, obs):
         """The slot's captured execute for ``key``, bound to ``curves``
-        (captured on first use), or None if the arena cannot hold a new one
-        or the batch does not fit the captured one."""
+        (captured when the shape runs for the second time), or None for the
+        first batch of a shape, if the arena cannot hold a new capture or if
+        the batch does not fit the captured one."""
         entry = slot.graphs.get(key)
         if entry is not None:
             try:
@@ -437,6 +446,12 @@ class GLSServer:
                 return None
             slot.graphs.move_to_end(key)
             return entry
+        if key not in self._shapes_seen:
+            self._shapes_seen[key] = True
+            if len(self._shapes_seen) > self.SHAPES_REMEMBERED:
+                self._shapes_seen.popitem(last=False)
+            return None
+        self._shapes_seen.move_to_end(key)
         mode, k, (f0, df, nf), nreq = key
         cap = max(self.max_batch_points, obs)
         plan = Plan.lomb(curves, self.sigma, self.m, mode, 0.0, 0.0, _lib.INPUT_RAW,

@@ -214,6 +214,33 @@ int gls_plan_grid_size(const gls_plan* plan, int32_t i, int64_t* n);
  * (zeros for the direct method). */
 int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n,
                             int32_t* n1, int32_t* n2, double* b);
+/* Tiles the plan uses for curve i's size bucket: c1 grid columns per pass-1
+ * workgroup, cr grid rows per pass-2 workgroup (zeros for the direct method). */
+int gls_plan_curve_tiles(const gls_plan* plan, int32_t i, int32_t* c1, int32_t* cr);
+
+/* ---- tiles of the NFFT passes (no counterpart in the reference, whose FFTs
+ * are cuFFT plans and whose gridding kernel has a fixed block size) --------
+ * A size bucket is a four-step split (n1, p2), n = n1 * 2^p2.  Plans take a
+ * bucket's tiles from a table measured on the device and compiled into the
+ * library (default rules for buckets it does not list).  gls_tile_set
+ * overrides them for plans created afterwards, process-wide: c1 = 0 or
+ * cr = 0 keeps the library's choice for that pass, both 0 remove the
+ * override.  GLS_E_UNSUPPORTED if the library has no kernel for the tile
+ * (a library built for tile sweeps has one for every candidate: c1 and cr
+ * powers of two up to 16, n1 * c1 <= 8192, 256 <= 2^p2 * cr <= 4096).
+ * gls_tile_get returns the tiles a plan made now would use. */
+int gls_tile_set(int32_t n1, int32_t p2, int32_t c1, int32_t cr);
+int gls_tile_get(int32_t n1, int32_t p2, int32_t* c1, int32_t* cr);
+/* A plan gives each curve the grid n = n1 * 2^p2 >= sigma * nf of least
+ * n x (cost per cell of its bucket): the time of the two passes per cell
+ * measured on the device (the same table), or a model (one pass over the
+ * cells plus 0.12 per FFT radix stage, scaled to the table) for buckets it
+ * does not list.  gls_grid_cost_set overrides a bucket's cost in
+ * picoseconds per cell for plans created afterwards (<= 0 removes the
+ * override); gls_grid_cost_get returns the cost in use and its source
+ * (0 model, 1 measured table, 2 override). */
+int gls_grid_cost_set(int32_t n1, int32_t p2, double ps_per_cell);
+int gls_grid_cost_get(int32_t n1, int32_t p2, double* ps_per_cell, int32_t* source);
 
 /* Runs the batch on `stream`: replaces lomb_scargle_async for every curve
  * (cuvarbase/lombscargle.py:579-689: nfft_adjoint_async twice + `lomb`, or

@@ -168,7 +168,8 @@ def test_graph_served_top5_matches_eager_and_oracle():
 
     sg, rg = served(True)
     se, re_ = served(False)
-    assert sg.graph_captures == 1 and sg.graph_replays == 3
+    # three batches of one shape: eager, captured, replayed
+    assert sg.graph_captures == 1 and sg.graph_replays == 2
     assert se.graph_captures == 0 and se.graph_replays == 0
     for i, (a, b, d) in enumerate(zip(rg, re_, data)):
         np.testing.assert_array_equal(a.top_power, b.top_power)

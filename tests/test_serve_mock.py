@@ -306,11 +306,11 @@ def test_tiny_budget_skips_warm_up(serve):
 
 
 def test_shared_grid_batches_replay_captured_graphs(serve):
-    """Requests on one linear grid: the first batch of a shape captures a
-    graph on its slot, later batches of the shape rebind the plan and replay
-    it (inputs at the captured row stride, descriptors uploaded before the
-    replay); top-k and whole periodograms match the oracle, and a batch of
-    another size is a new shape."""
+    """Requests on one linear grid: the first batch of a shape runs eagerly,
+    the second captures a graph on its slot, later batches of the shape
+    rebind the plan and replay it (inputs at the captured row stride,
+    descriptors uploaded before the replay); top-k and whole periodograms
+    match the oracle, and a batch of another size is a new shape."""
     rng = np.random.default_rng(11)
     grid = 0.05 + 0.01 * np.arange(300)
     data = []
@@ -324,7 +324,8 @@ def test_shared_grid_batches_replay_captured_graphs(serve):
         reqs = [srv.submit(*d, grid, k=k) for d in data]
         srv.drain()
         assert srv.batches_launched == 4          # 4 + 4 + 4 + 2 curves
-        assert srv.graph_captures == 2 and srv.graph_replays == 4
+        # shape 4: eager, captured, replayed; shape 2: seen once, eager
+        assert srv.graph_captures == 1 and srv.graph_replays == 2
         for r, d in zip(reqs, data):
             ref = og.gls_direct(*d, grid)
             if k:
@@ -340,7 +341,10 @@ def test_graph_entries_are_bounded_per_slot(serve):
     srv = serve.GLSServer(hbm_budget=1 << 24, slots=1, max_batch_curves=2,
                           max_batch_points=300, graphs_per_slot=2)
     used_before = srv.arena.used
-    for nf in (200, 210, 220, 200):
+    # each shape twice in a row (eager, then captured); the third capture
+    # evicts the least recently used; a shape seen before is captured again
+    # at once after its eviction
+    for nf in (200, 200, 210, 210, 220, 220, 200):
         grid = 0.05 + 0.01 * np.arange(nf)
         for _ in range(2):
             m = int(rng.integers(20, 60))
@@ -348,5 +352,6 @@ def test_graph_entries_are_bounded_per_slot(serve):
             srv.submit(t, np.sin(t) + rng.normal(0, 0.2, m), np.full(m, 0.2), grid)
         srv.drain()
     assert len(srv.slots[0].graphs) == 2 and srv.graph_captures == 4
+    assert srv.graph_replays == 4 and srv.batches_launched == 7
     assert srv.arena.used > used_before
     assert all(r.power is not None for r in srv.completed)
