@@ -51,7 +51,7 @@ def long_curve_oracle():
     return onfft.gls_nfft(t, y, dy, freqs[0], sigma=2.0, m=10)
 
 
-def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8):
+def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8, single_s=4.0):
     """Time the FP64 oracle port (numpy NFFT + GLS) on a bounded sample.
 
     Returns the baseline record and {curve index: oracle periodogram} for the
@@ -82,6 +82,16 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8):
         wall = time.perf_counter() - t0
     done = sum(r[0] for r in res)
     kept = {i: r[1] for i, r in zip(sample, res) if r[1] is not None}
+    # one core: the same port in this process over the sample's first curves,
+    # about single_s seconds of work
+    t0 = time.perf_counter()
+    one, ncur = 0, 0
+    for i in sample:
+        if time.perf_counter() - t0 >= single_s:
+            break
+        one += _cpu_worker((*data[i], freqs[i], False))[0]
+        ncur += 1
+    one_wall = time.perf_counter() - t0
     return {
         "value": done / wall,
         "unit": "evals/s",
@@ -91,6 +101,9 @@ def cpu_baseline(data, freqs, target_s=12.0, max_curves=400, seed=7, keep=8):
                   f"NumPy NFFT (sigma=2, m=8) + GLS, one curve per task on a "
                   f"{cores}-process pool",
         "wall_s": wall,
+        "single_core": {"value": one / one_wall, "unit": "evals/s", "cores": 1,
+                        "sample": f"the first {ncur} curves of that sample ({one:.3g} evals) "
+                                  "in one process"},
     }, kept
 
 

@@ -154,41 +154,22 @@ __global__ __launch_bounds__(NT) void dirsum_custom_kernel(const CurveGeom* __re
 // largest at the band edge where the lowest frequencies sit, reaches |dP| ~
 // 1e-4.  These few frequencies are recomputed by direct sums with FP64
 // phases and FP64 accumulation.
-// Grid (curve, group of 4 frequencies); each wave of a block takes one
-// frequency and its 64 lanes stride over the curve's points (FP64 phases,
-// FP64 sums), then reduce by shuffles.  Groups past the curve's low
-// frequencies exit at once.
+// Grid (curve, group of 4 frequencies), the groups strided over the
+// frequencies from kbegin on: each wave of a block takes one frequency and
+// its 64 lanes stride over the curve's points (FP64 phases, FP64 sums), then
+// reduce by shuffles.  A block stops at the first group past the curve's low
+// frequencies, so however many there are (fine grids have hundreds), every
+// one is covered.
 constexpr int kLowPerBlock = NT / kWave;
+constexpr int kLowGroups = kLowMax / kLowPerBlock;   // blocks per curve
 
-// WIDE: one frequency per 1024-thread block, all its lanes over the points
-// (long curves: 16 waves per frequency hide the FP64 latency)
-constexpr int kLowWideNT = 1024;
-template <int OUT, bool WIDE>
-__global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
-    const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
-    const CurveState* __restrict__ state, PointArrays pa, PlanParams pp,
-    void* __restrict__ out) {
-    constexpr int BT = WIDE ? kLowWideNT : NT;     // threads per block
-    __shared__ double red[BT / kWave][6];
-    constexpr int FPB = WIDE ? 1 : kLowPerBlock;   // frequencies per block
-    constexpr int LANES = WIDE ? BT : kWave;       // lanes per frequency
-    const int tid = threadIdx.x;
-    const int lane = tid % LANES;
-    const int k = blockIdx.y * FPB + tid / LANES;
-    const int c = list[blockIdx.x];
-    const CurveGeom& G = geom[c];
-    const CurveState& st = state[c];
-    // frequency k is low if it and every frequency before it span fewer than
-    // kLowCycles cycles over the baseline (the leading run of the grid)
-    const int first = blockIdx.y * FPB;
-    if (first >= min(G.nf, kLowMax)) return;
-    if (fabs(G.f0 + (double)first * G.df) * st.tspan >= kLowCycles) return;
-    bool active = k < min(G.nf, kLowMax);
-    for (int q = first; q <= k && active; ++q)
-        active = fabs(G.f0 + (double)q * G.df) * st.tspan < kLowCycles;
-    if (!active) return;   // uniform per frequency: whole waves (or the block) exit
+// the six FP64 sums of frequency k over the curve's points and its power
+template <int OUT, bool WIDE, int BT, int LANES>
+__device__ __forceinline__ void lowfreq_one(const CurveGeom& G, const CurveState& st,
+                                            const PointArrays& pa, const PlanParams& pp, int k,
+                                            int tid, int lane, int N, double (*red)[6],
+                                            void* __restrict__ out) {
     const double f = G.f0 + (double)k * G.df;
-    const int N = G.n_obs;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = lane; j < N; j += LANES) {
         const double x = f * pa.dt[G.pts + j];
@@ -212,12 +193,47 @@ __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
         if (tid == 0)
             for (int w = 1; w < BT / kWave; ++w)
                 for (int q = 0; q < 6; ++q) acc[q] += red[w][q];
+        __syncthreads();   // red is reused by the block's next frequency
     }
     if (lane == 0) {
         const double P = gls_power(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], st.yy,
                                    st.ybar, pp.mode, pp.reg_c, pp.reg_s);
         if (OUT == 0) ((float*)out)[G.out + k] = (float)P;
         else ((double*)out)[G.out + k] = P;
+    }
+}
+
+// WIDE: one frequency per 1024-thread block, all its lanes over the points
+// (long curves: 16 waves per frequency hide the FP64 latency)
+constexpr int kLowWideNT = 1024;
+template <int OUT, bool WIDE>
+__global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
+    const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
+    const CurveState* __restrict__ state, PointArrays pa, PlanParams pp, int kbegin,
+    void* __restrict__ out) {
+    constexpr int BT = WIDE ? kLowWideNT : NT;     // threads per block
+    __shared__ double red[BT / kWave][6];
+    constexpr int FPB = WIDE ? 1 : kLowPerBlock;   // frequencies per block
+    constexpr int LANES = WIDE ? BT : kWave;       // lanes per frequency
+    const int tid = threadIdx.x;
+    const int lane = tid % LANES;
+    const int c = list[blockIdx.x];
+    const CurveGeom& G = geom[c];
+    const CurveState& st = state[c];
+    const int N = G.n_obs;
+    for (int first = kbegin + (int)blockIdx.y * FPB; first < G.nf;
+         first += (int)gridDim.y * FPB) {
+        // frequency k is low if it and every frequency before it span fewer
+        // than kLowCycles cycles over the baseline (the leading run of the grid)
+        if (fabs(G.f0 + (double)first * G.df) * st.tspan >= kLowCycles) return;
+        const int k = first + tid / LANES;
+        bool active = k < G.nf;
+        for (int q = first; q <= k && active; ++q)
+            active = fabs(G.f0 + (double)q * G.df) * st.tspan < kLowCycles;
+        // uniform per frequency: whole waves (or the block) exit; later groups
+        // of this block lie further along the grid, past the run as well
+        if (!active) return;
+        lowfreq_one<OUT, WIDE, BT, LANES>(G, st, pa, pp, k, tid, lane, N, red, out);
     }
 }
 
@@ -303,17 +319,20 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
                           WideAux* aux, hipStream_t s) {
     if (nsmall > 0) {
-        const dim3 grid(nsmall, (kLowMax + kLowPerBlock - 1) / kLowPerBlock);
+        const dim3 grid(nsmall, kLowGroups);
         if (out_f64)
             hipLaunchKernelGGL((lowfreq_kernel<1, false>), grid, dim3(NT), 0, s, d_geom, small,
-                               state, pa, pp, power);
+                               state, pa, pp, 0, power);
         else
             hipLaunchKernelGGL((lowfreq_kernel<0, false>), grid, dim3(NT), 0, s, d_geom, small,
-                               state, pa, pp, power);
+                               state, pa, pp, 0, power);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (nwide > 0 && aux != nullptr) {
+        // the first kLowMax frequencies split over kLowSplit workgroups each
+        // (one accumulator slot per frequency in the record), any further
+        // low ones by whole workgroups
         const dim3 grid(nwide, kLowMax, kLowSplit);
         if (out_f64)
             hipLaunchKernelGGL((lowfreq_split_kernel<1>), grid, dim3(kLowWideNT), 0, s, d_geom,
@@ -321,14 +340,18 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
         else
             hipLaunchKernelGGL((lowfreq_split_kernel<0>), grid, dim3(kLowWideNT), 0, s, d_geom,
                                wide, state, pa, pp, aux, power);
-    } else if (nwide > 0) {
-        const dim3 grid(nwide, kLowMax);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nwide > 0) {
+        const int kbegin = aux != nullptr ? kLowMax : 0;
+        const dim3 grid(nwide, aux != nullptr ? kLowGroups : kLowMax);
         if (out_f64)
             hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
-                               wide, state, pa, pp, power);
+                               wide, state, pa, pp, kbegin, power);
         else
             hipLaunchKernelGGL((lowfreq_kernel<0, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
-                               wide, state, pa, pp, power);
+                               wide, state, pa, pp, kbegin, power);
     }
     return hipGetLastError();
 }

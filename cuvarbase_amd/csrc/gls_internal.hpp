@@ -198,9 +198,9 @@ hipError_t launch_dirsum(const CurveGeom* d_geom, const CurveState* state,
                          const PlanParams& pp, void* power, int out_f64,
                          hipStream_t s);
 // direct FP64 sums for a curve's lowest frequencies, f_k (t_max - t_min) <
-// kLowCycles (at most kLowMax of them), overwriting the NFFT powers there;
-// with aux (one WideAux per wide curve) a wide curve's frequencies are each
-// summed by kLowSplit workgroups
+// kLowCycles (all of them), overwriting the NFFT powers there; with aux (one
+// WideAux per wide curve) a wide curve's first kLowMax such frequencies are
+// each summed by kLowSplit workgroups, any further ones by one workgroup
 hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsmall,
                           const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,

@@ -172,8 +172,9 @@ class Request:
         h, self._hold = self._hold, None
         if h is not None:
             if self.top_power is not None:
-                # keep what the peaks resolve to before their memory goes back
-                _ = self.top_freq, self.fap
+                # the peaks outlive their memory as copies (k values each);
+                # their frequencies and false-alarm probabilities are still
+                # computed from these on first access
                 self.top_power = np.array(self.top_power)
                 self.top_index = np.array(self.top_index)
             self.power = None
@@ -298,6 +299,10 @@ class GLSServer:
         self.window_beta = float(cfg["window_beta"])
         self.mode = mode
         self.max_batch_evals = int(max_batch_evals)
+        # batch size in evaluations now: halved while a batch does not fit
+        # the free memory of an idle server, doubled back (up to
+        # max_batch_evals) by every batch that launches
+        self._evals_cap = self.max_batch_evals
         self.max_batch_curves = int(max_batch_curves)
         self.max_batch_points = max(int(max_batch_points), 1)
         self.graph_batches = bool(graph_batches)
@@ -401,7 +406,7 @@ class GLSServer:
             if self._class(r) != cls:
                 rest.append(self.queue.popleft())
                 continue
-            if reqs and (ev + r.nf > self.max_batch_evals
+            if reqs and (ev + r.nf > self._evals_cap
                          or pts + len(r.t) > self.max_batch_points):
                 break
             reqs.append(self.queue.popleft())
@@ -612,8 +617,9 @@ class GLSServer:
                             f"request {reqs[0].id} needs more device or result memory than "
                             f"the server budget ({self.arena.nbytes} bytes)")
                     # retry with a smaller batch next time
-                    self.max_batch_evals = max(1, self.max_batch_evals // 2)
+                    self._evals_cap = max(1, self._evals_cap // 2)
                 break
+            self._evals_cap = min(self.max_batch_evals, 2 * self._evals_cap)
         return len(self.completed) - before
 
     @property

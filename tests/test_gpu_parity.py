@@ -522,47 +522,63 @@ def test_entry_point_smoke():
 # one frequency count per pass-1 column length that the grid chooser reaches
 # (every size of csrc/sizes.hpp except 72, 648, 1080, 1152 and 2160, which a
 # cheaper split of the same grid always beats), at sigma = 2
-_CATALOGUE_NF = [512, 640, 768, 800, 864, 1152, 1280, 1536, 2560, 2592, 4096, 4608, 6400,
-                 6912, 8192, 12288, 12800, 12960, 13824, 20480, 24576, 40960, 41472, 65536,
-                 73728, 98304, 102400, 110592, 131072, 1658880]
+def _catalogue_buckets():
+    """Every pass-1 column length with rows of 64 cells, and rows of 256,
+    1024, 2048 and 4096 cells; all grid sizes distinct."""
+    from cuvarbase_amd import build
+
+    out = [(n1, 6) for n1 in build.pass1_sizes()]
+    out += [(2304, 8), (405, 10), (1280, 11), (81, 12)]
+    assert len({n1 << p2 for n1, p2 in out}) == len(out)
+    return out
 
 
 @pytest.mark.parametrize("window", ["gaussian", "es"])
 def test_catalogue_sizes_against_direct_sums(window):
-    """One batch with a curve for (nearly) every pass-1 column length, so
-    every pass-1 kernel and the pass-2 row lengths 64 and 1024 run, including
-    partial row blocks (odd column lengths); powers against FP64 direct sums
-    at the leading frequencies (the low-frequency fix-up) and 48 random ones."""
+    """One batch with a curve on every pass-1 column length (and the pass-2
+    row lengths 64 to 4096), including partial row blocks (odd column
+    lengths): each bucket is made the cheapest grid while the batch is
+    planned (gls_grid_cost_set), so each curve lands on the smallest of them
+    that holds it, its own; powers against FP64 direct sums at the leading
+    frequencies (the low-frequency fix-up) and 48 random ones."""
     from cuvarbase_amd import engine, tuning
     from cuvarbase_amd import _lib
 
+    if window == "es":
+        sigma = 1.35
+        kw = dict(sigma=sigma, m=5, window=_lib.WINDOW_ES,
+                  window_beta=tuning.es_beta(1.35, 5, 1.05))
+        tol = 2e-5
+    else:
+        sigma = 2.0
+        kw = dict(sigma=sigma, m=8)
+        tol = 1e-5
+    buckets = _catalogue_buckets()
     rng = np.random.default_rng(77)
     data, freqs = [], []
-    for nf in _CATALOGUE_NF:
+    for n1, p2 in buckets:
+        nf = int((n1 << p2) / sigma) - 1
         T = float(rng.uniform(50, 500))
         t = np.sort(rng.uniform(0, T, 80))
         y = 0.4 * np.sin(2 * np.pi * t / rng.uniform(0.3, 20)) + rng.normal(0, 0.2, t.size)
         data.append((t, y, np.full(t.size, 0.2)))
         freqs.append((1 + np.arange(nf)) / (5 * T))
-    if window == "es":
-        kw = dict(sigma=1.35, m=5, window=_lib.WINDOW_ES,
-                  window_beta=tuning.es_beta(1.35, 5, 1.05))
-        tol, ncols = 2e-5, 12     # (the smaller sigma lands on fewer sizes)
-    else:
-        kw = dict(sigma=2.0, m=8)
-        tol, ncols = 1e-5, 25
-    b = engine.LombBatch(data, freqs, **kw)
+    for n1, p2 in buckets:
+        engine.set_grid_cost(n1, p2, 1e-6)
+    try:
+        b = engine.LombBatch(data, freqs, **kw)
+    finally:
+        for n1, p2 in buckets:
+            engine.set_grid_cost(n1, p2, 0.0)
+    got = [(g[1], g[2].bit_length() - 1) for g in (b.plan.geometry(i) for i in range(len(data)))]
+    assert got == buckets
     P = b.run()
-    n1s = {b.plan.geometry(i)[1] for i in range(len(data))}
-    n2s = {b.plan.geometry(i)[2] for i in range(len(data))}
-    print(f"{window}: {len(n1s)} column lengths, row lengths {sorted(n2s)}")
-    assert len(n1s) >= ncols
     worst = 0.0
     for (t, y, dy), f, p in zip(data, freqs, P):
         sel = np.concatenate([np.arange(8), rng.choice(len(f), size=48, replace=False)])
         ref = og.gls_direct(t, y, dy, f[sel])
         worst = max(worst, float(np.max(np.abs(p[sel].astype(np.float64) - ref))))
-    print(f"{window}: max|dP| = {worst:.3e}")
+    print(f"{window}: {len(buckets)} buckets, max|dP| = {worst:.3e}")
     assert worst < tol
 
 
@@ -798,3 +814,42 @@ def test_fp32_coordinates_precision_study():
     assert res[_lib.PRECISION_FP64_COORDS][0] < TOL
     assert res[_lib.PRECISION_FP64_COORDS][1] == kbest
     assert res[_lib.PRECISION_FP32][0] > 10 * res[_lib.PRECISION_FP64_COORDS][0]
+
+
+def test_fine_grid_low_frequencies_all_direct():
+    """A grid fine enough that more than 256 leading frequencies span fewer
+    than 3 cycles over the baseline (df = 1/(120 T): ~360 of them): every one
+    is recomputed by FP64 direct sums, for a short curve and for a curve of
+    more than 16,384 points (whose first 256 are split over workgroups and the
+    rest summed whole).  Run at a deliberately coarse NFFT (Gaussian sigma =
+    1.25, m = 2, |dP| ~1e-3) so that only the direct sums can meet the
+    tolerance there."""
+    from cuvarbase_amd import engine
+
+    rng = np.random.default_rng(41)
+    data, freqs = [], []
+    for n, T in ((60, 50.0), (20_000, 80.0)):
+        t = np.sort(rng.uniform(0, T, n))
+        y = np.sin(2 * np.pi * t / 9.0) + rng.normal(0, 0.5, n)
+        dy = rng.uniform(0.3, 0.6, n)
+        data.append((t, y, dy))
+        df = 1.0 / (120 * T)
+        freqs.append(df * (1 + np.arange(2000)))
+    out = engine.LombBatch(data, freqs, sigma=1.25, m=2).run()
+    for i, ((t, y, dy), f) in enumerate(zip(data, freqs)):
+        fT = f * (t.max() - t.min())
+        nlow = int(np.sum(fT < 3.0))
+        assert nlow > 300
+        # from one cycle over the baseline on: below it the GLS is so badly
+        # conditioned (CC SS - CS^2 ~ (f T)^4) that the FP32 weights alone move
+        # the power, whoever sums it
+        lo = int(np.sum(fT < 1.0))
+        sel = np.arange(lo, nlow + 200)
+        Pd = og.gls_direct(t, y, dy, f[sel])
+        d = np.abs(out[i][sel] - Pd)
+        err_low, err_next = np.max(d[:nlow - lo]), np.max(d[nlow - lo:])
+        err_256 = np.max(d[256 - lo:nlow - lo])
+        print(f"curve {i} n={len(t)}: {nlow} low frequencies, max |dP| {err_low:.2e} "
+              f"over {lo}..{nlow - 1} ({err_256:.2e} from 256 on; FP64 sums), "
+              f"{err_next:.2e} over the next 200 (coarse NFFT)")
+        assert err_low < 1e-5

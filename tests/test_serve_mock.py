@@ -355,3 +355,61 @@ def test_graph_entries_are_bounded_per_slot(serve):
     assert srv.graph_replays == 4 and srv.batches_launched == 7
     assert srv.arena.used > used_before
     assert all(r.power is not None for r in srv.completed)
+
+
+def test_batch_cap_recovers_after_a_batch_that_did_not_fit(serve):
+    """A batch that does not fit an idle server halves the batch size for the
+    next attempt; every batch that launches doubles it back, up to
+    max_batch_evals, so one tight moment does not shrink batches for good."""
+    data, freqs = _requests(24)
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=1, max_batch_evals=4000)
+    real = srv._launch
+    sizes = []
+
+    def tight(slot, reqs):
+        ev = sum(r.nf for r in reqs)
+        if ev > 1500 and len(reqs) > 1:
+            return False          # as if the arena could not hold it
+        sizes.append(ev)
+        return real(slot, reqs)
+
+    srv._launch = tight
+    first = [srv.submit(*d, f) for d, f in zip(data, freqs)]
+    srv.drain()
+    assert max(sizes) <= 1500 and srv.max_batch_evals == 4000
+    for r, d, f in zip(first, data, freqs):
+        np.testing.assert_allclose(r.power, og.gls_direct(*d, f), atol=1e-6)
+    srv._launch = real
+    sizes.clear()
+    second = [srv.submit(*d, f) for d, f in zip(data, freqs)]
+    n0 = srv.batches_launched
+    srv.drain()
+    assert srv._evals_cap == 4000
+    assert srv.batches_launched - n0 <= 4     # batches of up to 4000 evaluations again
+    for r, d, f in zip(second, data, freqs):
+        np.testing.assert_allclose(r.power, og.gls_direct(*d, f), atol=1e-6)
+
+
+def test_peaks_resolve_after_release(serve):
+    """Releasing a zero-copy top-k result keeps copies of its k peaks; their
+    frequencies and false-alarm probabilities are computed on first access,
+    after the release, from those copies (not at release time, which the
+    serving loop would pay for every request)."""
+    from cuvarbase_amd.lombscargle import fap_baluev
+
+    data, freqs = _requests(6, seed=21)
+    srv = serve.GLSServer(hbm_budget=1 << 22, slots=2, max_batch_evals=700, zero_copy=True,
+                          result_buffer_bytes=1 << 14)
+    reqs = [srv.submit(*d, f, k=3) for d, f in zip(data, freqs)]
+    srv.drain()
+    for r in reqs:
+        assert r._fap is None and r._top_freq is None
+        r.release()
+    assert srv.host_arena.used == 0
+    for r, (t, y, dy), f in zip(reqs, data, freqs):
+        P = og.gls_direct(t, y, dy, f)
+        order = np.argsort(-P.astype(np.float32), kind="stable")[:3]
+        np.testing.assert_array_equal(r.top_index, order)
+        np.testing.assert_allclose(r.top_freq, f[order], rtol=1e-12)
+        want = fap_baluev(t - np.mean(t), dy, np.asarray(r.top_power, np.float64), f[-1])
+        np.testing.assert_allclose(r.fap, want, rtol=1e-12)
