@@ -55,6 +55,17 @@ constexpr int kMaxGrid = 1 << 24;   // largest grid n (cells)
 constexpr int kMinGrid = 1 << 10;   // smallest grid n
 
 // ---------------------------------------------------------------------------
+// Inter-pass grid layout (pass 1 writes, pass 2 reads): the rows k1 of a
+// grid in blocks of kRowBlock, each block stored row-block-major over the n2
+// columns, A[(k1 / R) R n2 + u2 R + k1 % R] (R = kRowBlock).  A pass-1 column
+// is then written as 128-byte segments and the rows of a pass-2 tile lie in
+// one contiguous run per row block (DRAM pages are opened once per tile,
+// not once per column).  A grid takes ceil(n1 / R) R n2 cells.
+constexpr int kRowBlock = 16;
+__host__ __device__ inline int64_t grid_cells(int n1, int p2) {
+    return (int64_t)((n1 + kRowBlock - 1) / kRowBlock * kRowBlock) << p2;
+}
+
 // per-curve plan data (host computed, uploaded once per plan)
 // ---------------------------------------------------------------------------
 struct CurveGeom {
@@ -62,7 +73,7 @@ struct CurveGeom {
     int64_t out;       // first output element
     int64_t pts;       // first point slot in the per-point workspace arrays
     int64_t bkt;       // first int of this curve's bucket table (2*(nb+1))
-    int64_t grid;      // first float2 of this curve's grids (ngrids * n)
+    int64_t grid;      // first float2 of this curve's grids (ngrids * grid_cells)
     double f0;
     double df;
     double b;          // Gaussian width in cells^2

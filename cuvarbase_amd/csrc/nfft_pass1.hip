@@ -214,7 +214,9 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
     // column FFT; its last stage writes straight from registers to HBM with
     // the inter-pass twiddle exp(2 pi i u2 k1 / n) (consecutive lanes hold
     // consecutive k1 of a column, so the stores coalesce)
-    const __amdgpu_buffer_rsrc_t out = grid_rsrc(grids + G.grid + (int64_t)gi * n, 8u * n);
+    const int64_t gs = grid_cells(N1, p2);   // cells of one grid
+    const __amdgpu_buffer_rsrc_t out = grid_rsrc(grids + G.grid + gi * gs, 8u * (uint32_t)gs);
+    const int rb = kRowBlock << p2;          // cells of one row block
     // (elo[0] = 1 exactly, so the product needs no special case; when the
     // output stride ns is a multiple of 64 a butterfly's outputs share one elo
     // entry and step through ehi, and the stride goes to the scalar offset)
@@ -226,7 +228,18 @@ __global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict_
             const int k1 = kb + q * ns;
             w = cmul(ehi[jj * NH + (k1 >> 6)], elo[jj * 64 + (k1 & 63)]);
         }
-        grid_store(out, 8u * (uint32_t)((a + jj) * N1 + kb), 8u * (uint32_t)(q * ns), cmul(v, w));
+        // row-block layout: when the output stride ns is a multiple of the
+        // block height, a butterfly's outputs share their in-block offset and
+        // step whole row blocks (the step goes to the scalar offset)
+        constexpr int R = kRowBlock;
+        if ((ns % R) == 0) {
+            grid_store(out, 8u * (uint32_t)((kb / R) * rb + (a + jj) * R + kb % R),
+                       8u * (uint32_t)(q * (ns / R) * rb), cmul(v, w));
+        } else {
+            const int k1 = kb + q * ns;
+            grid_store(out, 8u * (uint32_t)((k1 / R) * rb + (a + jj) * R + k1 % R), 0u,
+                       cmul(v, w));
+        }
     });
 }
 

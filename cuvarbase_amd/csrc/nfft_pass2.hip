@@ -45,14 +45,18 @@ __global__ __launch_bounds__(pass2_threads_for(P2, CR, OUT)) void pass2_kernel(
 
     // row FFTs of the tile (rows r0 .. r0+CR-1 of the three grids); the first
     // stage loads its inputs straight from HBM
-    const __amdgpu_buffer_rsrc_t src = grid_rsrc(grids + G.grid, 8u * (uint32_t)(NG * n));
+    const uint32_t gs = (uint32_t)grid_cells(n1, P2);   // cells of one grid
+    const __amdgpu_buffer_rsrc_t src = grid_rsrc(grids + G.grid, 8u * (NG * gs));
     build_twiddles<N2, NT>(T, tid);
     lds_fft_from<N2, B, BS, NT>(tile, T, tid, [&](int j, int q, int lr, int bb) -> float2 {
-        // column u2 = j + q lr.  Rows past n1 (a partial last block) read
-        // row n1 - 1 instead; their transforms are never emitted
+        // column u2 = j + q lr of row `row` in the row-block layout.  Rows past
+        // n1 (a partial last block) read row n1 - 1 instead; their transforms
+        // are never emitted
+        constexpr int R = kRowBlock;
         const int gi = bb / CR;
         const int row = min(r0 + bb - gi * CR, n1 - 1);
-        return grid_load(src, 8u * (uint32_t)(gi * n + (j + q * lr) * n1 + row));
+        return grid_load(src, 8u * (gi * gs + (uint32_t)((row / R) * (R * N2) + row % R)) +
+                                  8u * (uint32_t)((j + q * lr) * R));
     });
 
     const int K = G.nf;

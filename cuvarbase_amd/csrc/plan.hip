@@ -115,24 +115,12 @@ std::mutex g_tile_mu;
 std::map<std::pair<int, int>, BucketTiles> g_tile_override;   // gls_tile_set
 std::map<std::pair<int, int>, double> g_cost_override;        // gls_grid_cost_set
 
-// the measured table compiled in (tiles_gen.hpp), else the default rules
-BucketTiles table_tiles(int n1, int p2) {
-    BucketTiles t{default_pass1_cols(n1), default_pass2_rows(p2)};
-#define GLS_TILE_ROW(N, P, C, R, PS) \
-    if (n1 == N && p2 == P) t = BucketTiles{C, R};
-    GLS_TILE_TABLE(GLS_TILE_ROW)
-#undef GLS_TILE_ROW
-    return t;
-}
+constexpr int kNumRowSizes = kMaxLog2Row - kMinLog2Row + 1;
 
-// measured time of the two passes per grid cell (picoseconds) of bucket
-// (n1, p2) at its tiles, from the table compiled in; < 0 if not measured
-double table_cost(int n1, int p2) {
-#define GLS_COST_ROW(N, P, C, R, PS) \
-    if (n1 == N && p2 == P) return PS;
-    GLS_TILE_TABLE(GLS_COST_ROW)
-#undef GLS_COST_ROW
-    return -1.0;
+int size_index(int n1) {
+    for (int i = 0; i < kNumPass1Sizes; ++i)
+        if (kPass1Sizes[i] == n1) return i;
+    return -1;
 }
 
 // model cost per cell of an unmeasured bucket: one pass over the cells plus
@@ -141,24 +129,58 @@ double model_cost(int n1, int p2) {
     return GLS_MODEL_PS_PER_UNIT * (1.0 + 0.12 * (radix_stages(n1) + radix_stages(1 << p2)));
 }
 
-// cost per cell of every bucket, looked up once per plan: the caller's
-// override (gls_grid_cost_set), else the measured table, else the model
-struct GridCosts {
-    double ps[kMaxLog2Row - kMinLog2Row + 1][kNumPass1Sizes];
-    GridCosts() {
-        std::map<std::pair<int, int>, double> ov;
-        {
-            std::lock_guard<std::mutex> lk(g_tile_mu);
-            ov = g_cost_override;
-        }
+// The measured table compiled in (tiles_gen.hpp) as arrays over (p2,
+// catalogue position), built on first use: each bucket's tiles (default
+// rules where the table has no row) and cost per cell (the model where it
+// has none), so that planning does no table scans.
+struct TableIndex {
+    BucketTiles tiles[kNumRowSizes][kNumPass1Sizes];
+    double ps[kNumRowSizes][kNumPass1Sizes];
+    bool measured[kNumRowSizes][kNumPass1Sizes];
+    TableIndex() {
         for (int p2 = kMinLog2Row; p2 <= kMaxLog2Row; ++p2)
             for (int i = 0; i < kNumPass1Sizes; ++i) {
                 const int n1 = kPass1Sizes[i];
-                auto it = ov.find({n1, p2});
-                double c = it != ov.end() ? it->second : table_cost(n1, p2);
-                if (c < 0) c = model_cost(n1, p2);
-                ps[p2 - kMinLog2Row][i] = c;
+                tiles[p2 - kMinLog2Row][i] = BucketTiles{default_pass1_cols(n1),
+                                                         default_pass2_rows(p2)};
+                ps[p2 - kMinLog2Row][i] = model_cost(n1, p2);
+                measured[p2 - kMinLog2Row][i] = false;
             }
+#define GLS_TILE_ROW(N, P, C, R, PS) set(N, P, C, R, PS);
+        GLS_TILE_TABLE(GLS_TILE_ROW)
+#undef GLS_TILE_ROW
+    }
+    void set(int n1, int p2, int c1, int cr, double cost) {
+        const int i = size_index(n1);
+        if (i < 0 || p2 < kMinLog2Row || p2 > kMaxLog2Row) return;
+        tiles[p2 - kMinLog2Row][i] = BucketTiles{c1, cr};
+        ps[p2 - kMinLog2Row][i] = cost;
+        measured[p2 - kMinLog2Row][i] = true;
+    }
+};
+const TableIndex& table_index() {
+    static const TableIndex index;   // thread-safe initialisation on first use
+    return index;
+}
+
+BucketTiles table_tiles(int n1, int p2) {
+    const int i = size_index(n1);
+    if (i < 0 || p2 < kMinLog2Row || p2 > kMaxLog2Row)
+        return BucketTiles{default_pass1_cols(n1), default_pass2_rows(p2)};
+    return table_index().tiles[p2 - kMinLog2Row][i];
+}
+
+// cost per cell of every bucket, looked up once per plan: the caller's
+// override (gls_grid_cost_set), else the measured table, else the model
+struct GridCosts {
+    double ps[kNumRowSizes][kNumPass1Sizes];
+    GridCosts() {
+        std::memcpy(ps, table_index().ps, sizeof(ps));
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        for (const auto& kv : g_cost_override) {
+            const int i = size_index(kv.first.first);
+            if (i >= 0) ps[kv.first.second - kMinLog2Row][i] = kv.second;
+        }
     }
 };
 
@@ -262,7 +284,7 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         int64_t acc = 0;   // grid elements of the open entry
         for (int32_t ci : kv.second) {
             CurveGeom& G = P->geom[ci];
-            const int64_t cells = (int64_t)ngrids * G.n;
+            const int64_t cells = (int64_t)ngrids * grid_cells(G.n1, G.p2);
             if (chunk > 0 && B.count > 0 && (acc + cells) * (int64_t)sizeof(float2) > chunk) {
                 P->buckets.push_back(B);
                 B.first = (int64_t)bcurves.size();
@@ -945,8 +967,11 @@ int gls_grid_cost_set(int32_t n1, int32_t p2, double ps_per_cell) {
 }
 
 int gls_grid_cost_get(int32_t n1, int32_t p2, double* ps_per_cell, int32_t* source) {
-    double c = -1.0;
-    int src = 0;
+    const int i = size_index(n1);
+    if (i < 0 || p2 < kMinLog2Row || p2 > kMaxLog2Row)
+        return fail(GLS_E_INVALID, "(n1, p2) is not a grid size of the catalogue");
+    double c = table_index().ps[p2 - kMinLog2Row][i];
+    int src = table_index().measured[p2 - kMinLog2Row][i] ? 1 : 0;
     {
         std::lock_guard<std::mutex> lk(g_tile_mu);
         auto it = g_cost_override.find({n1, p2});
@@ -954,11 +979,6 @@ int gls_grid_cost_get(int32_t n1, int32_t p2, double* ps_per_cell, int32_t* sour
             c = it->second;
             src = 2;
         }
-    }
-    if (src == 0) {
-        c = table_cost(n1, p2);
-        src = c >= 0 ? 1 : 0;
-        if (c < 0) c = model_cost(n1, p2);
     }
     if (ps_per_cell) *ps_per_cell = c;
     if (source) *source = src;
