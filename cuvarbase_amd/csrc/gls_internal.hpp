@@ -58,10 +58,16 @@ constexpr int kMinGrid = 1 << 10;   // smallest grid n
 // Inter-pass grid layout (pass 1 writes, pass 2 reads): the rows k1 of a
 // grid in blocks of kRowBlock, each block stored row-block-major over the n2
 // columns, A[(k1 / R) R n2 + u2 R + k1 % R] (R = kRowBlock).  A pass-1 column
-// is then written as 128-byte segments and the rows of a pass-2 tile lie in
+// is then written as R x 8-byte segments and the rows of a pass-2 tile lie in
 // one contiguous run per row block (DRAM pages are opened once per tile,
-// not once per column).  A grid takes ceil(n1 / R) R n2 cells.
-constexpr int kRowBlock = 16;
+// not once per column).  A grid takes ceil(n1 / R) R n2 cells.  R = 8 (the
+// pass-2 tile height of the survey's main buckets) measured 4% faster in pass
+// 2 than R = 16 on the survey batch, with pass 1 unchanged.
+#ifndef GLS_ROW_BLOCK
+#define GLS_ROW_BLOCK 8
+#endif
+constexpr int kRowBlock = GLS_ROW_BLOCK;
+static_assert((kRowBlock & (kRowBlock - 1)) == 0, "the row block is a power of two");
 __host__ __device__ inline int64_t grid_cells(int n1, int p2) {
     return (int64_t)((n1 + kRowBlock - 1) / kRowBlock * kRowBlock) << p2;
 }
