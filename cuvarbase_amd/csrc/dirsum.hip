@@ -204,9 +204,13 @@ __device__ __forceinline__ void lowfreq_one(const CurveGeom& G, const CurveState
 }
 
 // WIDE: one frequency per 1024-thread block, all its lanes over the points
-// (long curves: 16 waves per frequency hide the FP64 latency)
+// (long curves: 16 waves per frequency hide the FP64 latency).
 constexpr int kLowWideNT = 1024;
-template <int OUT, bool WIDE>
+// LOOP: the block strides on over the grid (gridDim.y groups apart) until it
+// passes the curve's low run; without it, one group per block (fewer live
+// registers: the common case, launched with enough groups for kLowMax
+// frequencies, and LOOP launches take any frequencies after those)
+template <int OUT, bool WIDE, bool LOOP>
 __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
     const CurveGeom* __restrict__ geom, const int32_t* __restrict__ list,
     const CurveState* __restrict__ state, PointArrays pa, PlanParams pp, int kbegin,
@@ -220,11 +224,11 @@ __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
     const int c = list[blockIdx.x];
     const CurveGeom& G = geom[c];
     const CurveState& st = state[c];
-    const int N = G.n_obs;
-    for (int first = kbegin + (int)blockIdx.y * FPB; first < G.nf;
-         first += (int)gridDim.y * FPB) {
+    int first = kbegin + (int)blockIdx.y * FPB;
+    do {
         // frequency k is low if it and every frequency before it span fewer
         // than kLowCycles cycles over the baseline (the leading run of the grid)
+        if (first >= G.nf) return;
         if (fabs(G.f0 + (double)first * G.df) * st.tspan >= kLowCycles) return;
         const int k = first + tid / LANES;
         bool active = k < G.nf;
@@ -233,8 +237,9 @@ __global__ __launch_bounds__(WIDE ? kLowWideNT : NT) void lowfreq_kernel(
         // uniform per frequency: whole waves (or the block) exit; later groups
         // of this block lie further along the grid, past the run as well
         if (!active) return;
-        lowfreq_one<OUT, WIDE, BT, LANES>(G, st, pa, pp, k, tid, lane, N, red, out);
-    }
+        lowfreq_one<OUT, WIDE, BT, LANES>(G, st, pa, pp, k, tid, lane, G.n_obs, red, out);
+        first += (int)gridDim.y * FPB;
+    } while (LOOP);
 }
 
 // Wide curves with a WideAux record: grid (wide curve, frequency, part);
@@ -318,21 +323,27 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
                           const int32_t* wide, int nwide, const CurveState* state,
                           PointArrays pa, const PlanParams& pp, void* power, int out_f64,
                           WideAux* aux, hipStream_t s) {
+    // the first kLowMax frequencies: one group per block; any further ones
+    // (fine grids): a few looping blocks per curve, which exit at once when
+    // the low run ends before kLowMax
+    constexpr int kMoreGroups = 4;
+#define GLS_LOW(OUTV, W, L, GRID, NTH, LIST, KB)                                                \
+    hipLaunchKernelGGL((lowfreq_kernel<OUTV, W, L>), GRID, dim3(NTH), 0, s, d_geom, LIST, state, \
+                       pa, pp, KB, power)
     if (nsmall > 0) {
-        const dim3 grid(nsmall, kLowGroups);
-        if (out_f64)
-            hipLaunchKernelGGL((lowfreq_kernel<1, false>), grid, dim3(NT), 0, s, d_geom, small,
-                               state, pa, pp, 0, power);
-        else
-            hipLaunchKernelGGL((lowfreq_kernel<0, false>), grid, dim3(NT), 0, s, d_geom, small,
-                               state, pa, pp, 0, power);
+        if (out_f64) {
+            GLS_LOW(1, false, false, dim3(nsmall, kLowGroups), NT, small, 0);
+            GLS_LOW(1, false, true, dim3(nsmall, kMoreGroups), NT, small, kLowMax);
+        } else {
+            GLS_LOW(0, false, false, dim3(nsmall, kLowGroups), NT, small, 0);
+            GLS_LOW(0, false, true, dim3(nsmall, kMoreGroups), NT, small, kLowMax);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (nwide > 0 && aux != nullptr) {
         // the first kLowMax frequencies split over kLowSplit workgroups each
-        // (one accumulator slot per frequency in the record), any further
-        // low ones by whole workgroups
+        // (one accumulator slot per frequency in the record)
         const dim3 grid(nwide, kLowMax, kLowSplit);
         if (out_f64)
             hipLaunchKernelGGL((lowfreq_split_kernel<1>), grid, dim3(kLowWideNT), 0, s, d_geom,
@@ -342,17 +353,17 @@ hipError_t launch_lowfreq(const CurveGeom* d_geom, const int32_t* small, int nsm
                                wide, state, pa, pp, aux, power);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    } else if (nwide > 0) {
+        if (out_f64) GLS_LOW(1, true, false, dim3(nwide, kLowMax), kLowWideNT, wide, 0);
+        else GLS_LOW(0, true, false, dim3(nwide, kLowMax), kLowWideNT, wide, 0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
     if (nwide > 0) {
-        const int kbegin = aux != nullptr ? kLowMax : 0;
-        const dim3 grid(nwide, aux != nullptr ? kLowGroups : kLowMax);
-        if (out_f64)
-            hipLaunchKernelGGL((lowfreq_kernel<1, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
-                               wide, state, pa, pp, kbegin, power);
-        else
-            hipLaunchKernelGGL((lowfreq_kernel<0, true>), grid, dim3(kLowWideNT), 0, s, d_geom,
-                               wide, state, pa, pp, kbegin, power);
+        if (out_f64) GLS_LOW(1, true, true, dim3(nwide, kMoreGroups), kLowWideNT, wide, kLowMax);
+        else GLS_LOW(0, true, true, dim3(nwide, kMoreGroups), kLowWideNT, wide, kLowMax);
     }
+#undef GLS_LOW
     return hipGetLastError();
 }
 
