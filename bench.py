@@ -431,7 +431,7 @@ def main():
             srv.completed.clear()
             serving["rates"]["burst" if math.isinf(rate) else f"{rate:g}/s"] = {
                 k: st[k] for k in ("requests", "evals", "wall_s", "evals_per_s", "p50_ms",
-                                   "p99_ms", "batches")}
+                                   "p99_ms", "batches", "launch_host_ms")}
         t0 = time.perf_counter()
         ev = 0
         nps = min(100, len(sdata))

@@ -310,6 +310,7 @@ class GLSServer:
         self.graphs_per_slot = max(int(graphs_per_slot), 1)
         self.graph_captures = 0
         self.graph_replays = 0
+        self.launch_host_s = []             # host time of each batch launch (seconds)
         self._shapes_seen = OrderedDict()   # batch shapes run at least once (bounded)
         self.arena = DeviceArena(hbm_budget)
         self.zero_copy = bool(zero_copy)
@@ -346,6 +347,7 @@ class GLSServer:
         self.batches_launched = 0
         self.evals_done = 0
         self.graph_captures = self.graph_replays = 0
+        self.launch_host_s = []
         self._shapes_seen.clear()
 
     # -- client side ---------------------------------------------------------
@@ -602,7 +604,11 @@ class GLSServer:
             if slot.batch is not None:
                 continue
             reqs = self._take_batch()
-            if not self._launch(slot, reqs):
+            t_l = time.perf_counter()
+            ok = self._launch(slot, reqs)
+            t_l = time.perf_counter() - t_l
+            self.launch_host_s.append(t_l)
+            if not ok:
                 # not enough device memory now: requeue at the front and wait
                 # for in-flight batches to release theirs
                 self.queue.extendleft(reversed(reqs))
@@ -655,6 +661,8 @@ class GLSServer:
         taken it would."""
         reqs = []
         i, n = 0, len(data)
+        nl0 = len(self.launch_host_s)
+        b0, c0, r0 = self.batches_launched, self.graph_captures, self.graph_replays
         with self._gc_frozen():
             t0 = time.perf_counter()
             while i < n or self.busy:
@@ -679,9 +687,12 @@ class GLSServer:
             "evals_per_s": evals / wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "batches": self.batches_launched,
-            "graph_captures": self.graph_captures,
-            "graph_replays": self.graph_replays,
+            "batches": self.batches_launched - b0,
+            "graph_captures": self.graph_captures - c0,
+            "graph_replays": self.graph_replays - r0,
+            # host time of this stream's batch launches (plan, staging, copies,
+            # kernel launches): the serving loop's own cost per batch
+            "launch_host_ms": _quantiles_ms(self.launch_host_s[nl0:]),
             "requests_obj": reqs,
         }
 
@@ -718,6 +729,14 @@ class GLSServer:
                 flush()
             flush()
         return written
+
+
+def _quantiles_ms(v):
+    if not v:
+        return None
+    a = np.asarray(v) * 1e3
+    return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)),
+            "max": float(a.max()), "over_1ms": int(np.sum(a > 1.0)), "count": int(a.size)}
 
 
 def _fap(r, z, idx):
