@@ -266,6 +266,8 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--short-curves", type=int, default=100_000,
                     help="curves in the short-curve top-5 leg (0: skip)")
+    ap.add_argument("--fork-steps", type=int, default=5,
+                    help="extra steps timed with concurrent size buckets (0: skip)")
     ap.add_argument("--e2e", type=int, default=1,
                     help="time the end-to-end (host memory to host memory) batch (0: skip)")
     ap.add_argument("--spread", type=int, default=0,
@@ -346,6 +348,23 @@ def main():
             a[2] += by
     batch.plan.enable_timing(False)
     serial_ms = sum(v[1] for v in stage_acc.values()) / n_instr
+
+    # the optional concurrent launch order (gls_plan_set_fork: size buckets
+    # spread over the library's streams, so one bucket's VALU-bound pass 1
+    # can overlap another's HBM-bound pass 2), timed like the steps above;
+    # reported beside the headline, which keeps the serial default
+    fork_ms = None
+    if args.fork_steps > 0:
+        batch.plan.set_fork(True)
+        batch.execute(stream=s)
+        s.synchronize()
+        e0.record(s)
+        for _ in range(args.fork_steps):
+            batch.execute(stream=s)
+        e1.record(s)
+        e1.synchronize()
+        fork_ms = e1.time_since(e0) / args.fork_steps
+        batch.plan.set_fork(False)
 
     ncurves = batch.ncurves
     ws_bytes = batch.plan.workspace_bytes
@@ -431,7 +450,7 @@ def main():
             srv.completed.clear()
             serving["rates"]["burst" if math.isinf(rate) else f"{rate:g}/s"] = {
                 k: st[k] for k in ("requests", "evals", "wall_s", "evals_per_s", "p50_ms",
-                                   "p99_ms", "batches", "launch_host_ms")}
+                                   "p99_ms", "batches", "launch_host_ms", "slow_launches")}
         t0 = time.perf_counter()
         ev = 0
         nps = min(100, len(sdata))
@@ -615,6 +634,10 @@ def main():
         "end_to_end": e2e,
         "latency_ms": latency,
         "serial_launch_order_ms": serial_ms,
+        "concurrent_buckets": None if fork_ms is None else {
+            "ms_per_step": fork_ms, "evals_per_s": total_evals / (fork_ms / 1e3),
+            "note": "gls_plan_set_fork: size buckets on concurrent streams (opt-in); "
+                    "the headline value keeps the serial default"},
         "parity": parity,
         "serving": serving,
         "short_top5": short_leg,

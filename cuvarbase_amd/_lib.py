@@ -110,6 +110,7 @@ _PROTOS = {
     "gls_memcpy_h2d_async": [_vp, _vp, C.c_size_t, _vp],
     "gls_memcpy_d2h_async": [_vp, _vp, C.c_size_t, _vp],
     "gls_memcpy_d2d_async": [_vp, _vp, C.c_size_t, _vp],
+    "gls_memcpy_kernel_async": [_vp, _vp, C.c_size_t, _vp],
     "gls_memset_async": [_vp, C.c_int, C.c_size_t, _vp],
     "gls_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "gls_lomb_plan_create": [C.POINTER(_vp), C.POINTER(gls_curve), C.c_int32,

@@ -38,6 +38,7 @@ struct StageSpan {
 };
 struct StageRing {
     char* base = nullptr;
+    char* dev = nullptr;   // the ring's device address
     size_t head = 0;
     std::deque<StageSpan> inflight;
     hipEvent_t ev[kStageEvents] = {};
@@ -52,6 +53,14 @@ int stage_setup() {
     void* p = nullptr;
     GLS_HIP_TRY(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
     std::memset(p, 0, kStageBytes);   // fault the pages in now, not inside a copy
+    {
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            d = nullptr;
+        }
+        g_stage.dev = (char*)d;
+    }
     for (int i = 0; i < kStageEvents; ++i) {
         hipError_t e = hipEventCreateWithFlags(&g_stage.ev[i], hipEventDisableTiming);
         if (e != hipSuccess) {
@@ -76,6 +85,67 @@ void stage_teardown() {
     (void)hipHostFree(g_stage.base);
     g_stage = StageRing{};
 }
+// Copies by a kernel (loads and stores over the bus from the shader cores,
+// no DMA engine): the DMA path's hipMemcpyAsync has been measured holding the
+// calling thread for 7-9 ms now and then in a served stream of small batches,
+// with the device idle; a kernel copy is enqueued like any other launch.
+// Grid-stride loops over 16-byte words (both ends 16-byte aligned), the rest
+// byte by byte.
+constexpr int kCopyThreads = 256;
+__global__ __launch_bounds__(kCopyThreads) void copy16_kernel(const uint4* __restrict__ src,
+                                                              uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kCopyThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kCopyThreads)
+        dst[i] = src[i];
+}
+__global__ __launch_bounds__(kCopyThreads) void copy1_kernel(const unsigned char* __restrict__ src,
+                                                             unsigned char* __restrict__ dst,
+                                                             size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kCopyThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kCopyThreads)
+        dst[i] = src[i];
+}
+// src and dst: device addresses (device memory, or page-locked host memory
+// through its device pointer)
+hipError_t kernel_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const bool al = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0);
+    const size_t n16 = al ? bytes / 16 : 0;
+    if (n16 > 0) {
+        const size_t blocks = std::min<size_t>((n16 + kCopyThreads - 1) / kCopyThreads, 2048);
+        hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(kCopyThreads), 0, s,
+                           (const uint4*)src, (uint4*)dst, n16);
+    }
+    const size_t done = n16 * 16;
+    if (bytes > done) {
+        const size_t rem = bytes - done;
+        const size_t blocks = std::min<size_t>((rem + kCopyThreads - 1) / kCopyThreads, 1024);
+        hipLaunchKernelGGL(copy1_kernel, dim3((unsigned)blocks), dim3(kCopyThreads), 0, s,
+                           (const unsigned char*)src + done, (unsigned char*)dst + done, rem);
+    }
+    return hipGetLastError();
+}
+
+// the device address of p: itself for device memory, the mapped address for
+// page-locked host memory; nullptr for memory the device cannot reach
+void* device_address(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type == hipMemoryTypeDevice) return const_cast<void*>(p);
+    if (a.type == hipMemoryTypeHost) {
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, const_cast<void*>(p), 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return d;
+    }
+    return nullptr;
+}
+
 }  // namespace
 
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
@@ -109,7 +179,10 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
         release(R.inflight.begin());
     }
     std::memcpy(R.base + off, src, bytes);
-    hipError_t e = hipMemcpyAsync(dst, R.base + off, bytes, hipMemcpyHostToDevice, s);
+    // a kernel copy from the ring's mapped address (no DMA engine), or the
+    // DMA path when the ring has no device address
+    hipError_t e = R.dev ? kernel_copy(dst, R.dev + off, bytes, s)
+                         : hipMemcpyAsync(dst, R.base + off, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     const int ev = R.free_ev.back();
     e = hipEventRecord(R.ev[ev], s);
@@ -373,6 +446,20 @@ int gls_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream)
 int gls_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream) {
     if (bytes == 0) return GLS_OK;
     GLS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return GLS_OK;
+}
+
+int gls_memcpy_kernel_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return GLS_OK;
+    int rc = require_init();
+    if (rc) return rc;
+    if (!dst || !src) return fail(GLS_E_INVALID, "NULL pointer");
+    void* d = device_address(dst);
+    const void* sp = device_address(src);
+    if (!d || !sp)
+        return fail(GLS_E_INVALID, "gls_memcpy_kernel_async: both ends must be device memory "
+                                   "or page-locked host memory (gls_host_alloc)");
+    GLS_HIP_TRY(kernel_copy(d, sp, bytes, (hipStream_t)stream));
     return GLS_OK;
 }
 

@@ -311,6 +311,8 @@ class GLSServer:
         self.graph_captures = 0
         self.graph_replays = 0
         self.launch_host_s = []             # host time of each batch launch (seconds)
+        self.slow_launches = []             # step breakdown of launches over 2 ms
+        self._last_launch = None
         self._shapes_seen = OrderedDict()   # batch shapes run at least once (bounded)
         self.arena = DeviceArena(hbm_budget)
         self.zero_copy = bool(zero_copy)
@@ -348,6 +350,7 @@ class GLSServer:
         self.evals_done = 0
         self.graph_captures = self.graph_replays = 0
         self.launch_host_s = []
+        self.slow_launches = []
         self._shapes_seen.clear()
 
     # -- client side ---------------------------------------------------------
@@ -498,8 +501,10 @@ class GLSServer:
             out += r.nf
         k = max(r.k for r in reqs) if topk else 0
         nreq = len(reqs)
+        marks = [time.perf_counter()]
         key = None if custom else self._graph_key(reqs, mode, k)
         entry = self._graph_entry(slot, key, curves, obs) if key is not None else None
+        marks.append(time.perf_counter())
         if entry is not None:
             # a replay: the entry's blocks stay with it after the batch
             plan, blocks, dev = entry.plan, [], entry.blocks
@@ -524,6 +529,7 @@ class GLSServer:
                 blocks.append(blk)
             dev = blocks
             row = 8 * max(obs, 1)
+        marks.append(time.perf_counter())
         hblock = None
         if self.zero_copy:
             hblock = self.host_arena.alloc(8 * nreq * k if k else 4 * max(out, 1))
@@ -544,51 +550,63 @@ class GLSServer:
             rows[1, o:o + n] = r.y
             rows[2, o:o + n] = r.dy
             o += n
+        marks.append(time.perf_counter())
         s = slot.stream.handle
         p_in, p_pow, p_ws = (self.arena.ptr(b) for b in dev[:3])
         if entry is not None:
             # rows land at the captured row stride: three copies; then the
             # rebound descriptors, then the replay
             for q in range(3):
-                _lib.call("gls_memcpy_h2d_async", p_in + q * row,
-                          h_in.ctypes.data + q * 8 * obs, C.c_size_t(8 * obs), s)
+                _copy_in(p_in + q * row, h_in.ctypes.data + q * 8 * obs, 8 * obs, s)
             if entry.uses:
                 entry.graph.update(plan, slot.stream)
             entry.graph.launch(slot.stream)
             entry.uses += 1
             self.graph_replays += 1
         else:
-            _lib.call("gls_memcpy_h2d_async", p_in, h_in.ctypes.data, C.c_size_t(24 * obs), s)
+            _copy_in(p_in, h_in.ctypes.data, 24 * obs, s)
+            marks.append(time.perf_counter())
             if custom:
                 for r, o in zip(reqs, offsets):
                     slot.h_freq[o:o + r.nf] = r.freqs
                 p_fr = self.arena.ptr(dev[4])
-                _lib.call("gls_memcpy_h2d_async", p_fr, slot.h_freq.ctypes.data,
-                          C.c_size_t(8 * out), s)
+                _copy_in(p_fr, slot.h_freq.ctypes.data, 8 * out, s)
                 _lib.call("gls_lomb_execute_custom", plan.handle, p_in, p_in + row,
                           p_in + 2 * row, p_fr, p_pow, C.c_int32(0), p_ws, s)
             else:
                 _lib.call("gls_lomb_execute", plan.handle, p_in, p_in + row, p_in + 2 * row,
                           p_pow, p_ws, s)
+            marks.append(time.perf_counter())
         if k:
             p_top = self.arena.ptr(dev[3])
             _lib.call("gls_topk", plan.handle, p_ws, p_pow, None, C.c_int32(k), p_top,
                       p_top + 4 * nreq * k, s)
             if hblock is not None:
-                _lib.call("gls_memcpy_d2h_async", self.host_arena.ptr(hblock), p_top,
-                          C.c_size_t(8 * nreq * k), s)
+                _copy_out(self.host_arena.ptr(hblock), p_top, 8 * nreq * k, s)
             else:
-                _lib.call("gls_memcpy_d2h_async", h_out.ctypes.data, p_top,
-                          C.c_size_t(4 * nreq * k), s)
-                _lib.call("gls_memcpy_d2h_async", slot.h_idx.ctypes.data, p_top + 4 * nreq * k,
-                          C.c_size_t(4 * nreq * k), s)
+                _copy_out(h_out.ctypes.data, p_top, 4 * nreq * k, s)
+                _copy_out(slot.h_idx.ctypes.data, p_top + 4 * nreq * k, 4 * nreq * k, s)
         else:
             dst = self.host_arena.ptr(hblock) if hblock is not None else h_out.ctypes.data
-            _lib.call("gls_memcpy_d2h_async", dst, p_pow, C.c_size_t(4 * out), s)
+            _copy_out(dst, p_pow, 4 * out, s)
+        marks.append(time.perf_counter())
         slot.event.record(slot.stream)
+        marks.append(time.perf_counter())
         slot.batch = _Batch(reqs, None if entry is not None else plan, blocks, obs, out,
                             offsets, k=k, hblock=hblock)
         self.batches_launched += 1
+        # host time of the launch's steps (ms): graph lookup or capture, plan
+        # and memory, staging, copies and kernel launches
+        d = [float(x) for x in np.diff(marks) * 1e3]
+        info = {"requests": nreq, "points": obs, "evals": out,
+                "graph": "replay" if entry is not None else "eager",
+                "graph_ms": d[0], "plan_and_memory_ms": d[1], "staging_ms": d[2]}
+        if entry is None:
+            # eager: input copy, execute, result copies (and top-k), event record
+            info.update(h2d_ms=d[3], execute_ms=d[4], results_ms=d[5], event_ms=d[6])
+        else:
+            info.update(replay_ms=d[3], event_ms=d[4])
+        self._last_launch = info
         return True
 
     def poll(self):
@@ -608,6 +626,9 @@ class GLSServer:
             ok = self._launch(slot, reqs)
             t_l = time.perf_counter() - t_l
             self.launch_host_s.append(t_l)
+            if ok and t_l > 2e-3 and len(self.slow_launches) < 50:
+                self.slow_launches.append(dict(self._last_launch, total_ms=t_l * 1e3,
+                                               at_launch=len(self.launch_host_s) - 1))
             if not ok:
                 # not enough device memory now: requeue at the front and wait
                 # for in-flight batches to release theirs
@@ -662,6 +683,7 @@ class GLSServer:
         reqs = []
         i, n = 0, len(data)
         nl0 = len(self.launch_host_s)
+        ns0 = len(self.slow_launches)
         b0, c0, r0 = self.batches_launched, self.graph_captures, self.graph_replays
         with self._gc_frozen():
             t0 = time.perf_counter()
@@ -693,6 +715,7 @@ class GLSServer:
             # host time of this stream's batch launches (plan, staging, copies,
             # kernel launches): the serving loop's own cost per batch
             "launch_host_ms": _quantiles_ms(self.launch_host_s[nl0:]),
+            "slow_launches": self.slow_launches[ns0:],
             "requests_obj": reqs,
         }
 
@@ -729,6 +752,24 @@ class GLSServer:
                 flush()
             flush()
         return written
+
+
+# Copies between the page-locked staging / result memory and the arena: up to
+# KERNEL_COPY_MAX bytes by a copy kernel (gls_memcpy_kernel_async), which is
+# enqueued like any launch; the DMA path's hipMemcpyAsync was measured holding
+# the serving thread for 7-9 ms now and then on such small copies.  Larger
+# copies (bursts of large batches) keep the DMA engines' bandwidth.
+KERNEL_COPY_MAX = 16 << 20
+
+
+def _copy_in(dst, src, nbytes, stream):
+    fn = "gls_memcpy_kernel_async" if nbytes <= KERNEL_COPY_MAX else "gls_memcpy_h2d_async"
+    _lib.call(fn, dst, src, C.c_size_t(nbytes), stream)
+
+
+def _copy_out(dst, src, nbytes, stream):
+    fn = "gls_memcpy_kernel_async" if nbytes <= KERNEL_COPY_MAX else "gls_memcpy_d2h_async"
+    _lib.call(fn, dst, src, C.c_size_t(nbytes), stream)
 
 
 def _quantiles_ms(v):

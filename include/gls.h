@@ -188,6 +188,15 @@ int gls_host_free(void* ptr);
 int gls_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream);
 int gls_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream);
 int gls_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
+/* Copy by a kernel between device memory and page-locked host memory from
+ * gls_host_alloc (either end may be either), enqueued on `stream` like a
+ * launch.  No counterpart in the reference (whose copies are PyCUDA's
+ * set_async/get_async, cuvarbase/lombscargle.py:295-308): for small,
+ * latency-critical copies, since the DMA path (gls_memcpy_h2d_async) has
+ * been measured holding the calling thread for 7-9 ms now and then in a
+ * served stream of small batches.  GLS_E_INVALID for memory the device
+ * cannot reach (pageable host memory). */
+int gls_memcpy_kernel_async(void* dst, const void* src, size_t bytes, void* stream);
 int gls_memset_async(void* dst, int value, size_t bytes, void* stream);
 int gls_mem_info(size_t* free_bytes, size_t* total_bytes);
 

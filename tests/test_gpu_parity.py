@@ -853,3 +853,15 @@ def test_fine_grid_low_frequencies_all_direct():
               f"over {lo}..{nlow - 1} ({err_256:.2e} from 256 on; FP64 sums), "
               f"{err_next:.2e} over the next 200 (coarse NFFT)")
         assert err_low < 1e-5
+
+
+def test_calibrate_meets_its_target():
+    """tuning.calibrate: the cheapest (sigma, m) whose NFFT powers agree with
+    the direct sums to a tenth of the target on sampled survey curves."""
+    from cuvarbase_amd import synthetic, tuning
+
+    data, freqs = synthetic.survey(12, seed=3)
+    sigma, m, err = tuning.calibrate(data, freqs, tol=1e-4, probes=4, nprobe_freqs=128)
+    print(f"calibrate: sigma={sigma}, m={m}, measured max |dP| {err:.2e}")
+    assert err * 10.0 <= 1e-4
+    assert (sigma, m) in tuning.candidates(1e-4 * tuning.SAFETY / 10.0)
