@@ -20,7 +20,10 @@ namespace gls {
 
 namespace {
 
-constexpr int NT1 = 256;
+// threads of a pass-1 workgroup: 512 for tiles of 6144 cells or more (one
+// radix-16 butterfly of the first stage per thread at 8192 cells; two
+// columns of the long lengths), 256 below
+constexpr int pass1_threads(int n1, int c1) { return n1 * c1 >= 6144 ? 512 : 256; }
 
 // ---------------------------------------------------------------------------
 // pass 1: spread + column FFT + inter-pass twiddle
@@ -35,12 +38,13 @@ constexpr int NT1 = 256;
 // roundings and the sums do not depend on the order of the points.
 // C1: grid columns per workgroup (one LDS tile of N1 x C1 cells)
 template <int N1, int C1, bool FIX>
-__global__ __launch_bounds__(NT1) void pass1_kernel(const CurveGeom* __restrict__ bgeom,
+__global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const CurveGeom* __restrict__ bgeom,
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
                                                     float2* __restrict__ grids, int m,
                                                     int window, float es_beta, float ainv) {
     static_assert(pass1_tile_ok(N1, C1), "pass-1 tile out of range");
+    constexpr int NT1 = pass1_threads(N1, C1);
     constexpr int BS = lds_stride(C1);
     constexpr int NH = (N1 + 63) / 64;
     // (rounded up to whole 16-byte pairs so that it is cleared with 128-bit stores)
@@ -252,11 +256,13 @@ hipError_t pass1_dispatch(const CurveGeom* bgeom, int ncurves, int ngrids, int p
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
     const float ainv = 1.0f / ((float)pp.m + 0.5f);
     if (pp.spread_fixed)
-        hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), dim3((unsigned)nblocks), dim3(NT1), 0,
+        hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), dim3((unsigned)nblocks),
+                           dim3(pass1_threads(N1, C1)), 0,
                            s, bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
                            ainv);
     else
-        hipLaunchKernelGGL((pass1_kernel<N1, C1, false>), dim3((unsigned)nblocks), dim3(NT1), 0,
+        hipLaunchKernelGGL((pass1_kernel<N1, C1, false>), dim3((unsigned)nblocks),
+                           dim3(pass1_threads(N1, C1)), 0,
                            s, bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
                            ainv);
     return hipGetLastError();
