@@ -8,12 +8,12 @@ c1, the grid columns one pass-1 workgroup spreads and transforms, and cr, the
 grid rows one pass-2 workgroup transforms (csrc/tiles.hpp).  This script runs
 on the device the table is for, with the tile-sweep library (a kernel for every
 candidate tile).  For every bucket of the catalogue (1024 <= n <= 2^24) it
-plans a synthetic batch of identical curves with nf = n / sigma frequencies at
-the tuned NFFT parameters (sigma = 1.35, m = 5, ES window), forced onto that
-bucket, of about CELLS grid cells per grid and DENSITY points per cell (the
-survey workload's order).  It times pass 1 under every candidate c1 (pass 2 at
-its default tile), then pass 2 under every candidate cr (pass 1 at the chosen
-c1).  The times come from the plan's per-stage HIP events: the median of REPS
+plans a synthetic batch of curves with nf = n / sigma frequencies at the tuned
+NFFT parameters (sigma = 1.35, m = 5, ES window), forced onto that bucket, of
+about CELLS grid cells per grid, whose point counts follow the survey
+workload's (log-uniform over POINTS_MIN..POINTS_MAX).  It times pass 1 under
+every candidate c1 (pass 2 at its default tile), then pass 2 under every
+candidate cr (pass 1 at the chosen c1).  The times come from the plan's per-stage HIP events: the median of REPS
 executes after a warm-up.  A candidate replaces the default rule only when it
 is at least MARGIN faster.  The bucket's cost is the time of both passes at
 its tiles per grid cell; plans then give each curve the grid of least
@@ -35,7 +35,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SIGMA = 1.35
 M = 5
 CELLS = 1.5e8        # grid cells per grid of a timed batch
-DENSITY = 0.004      # points per grid cell
+POINTS_MIN, POINTS_MAX = 50, 5000   # points per curve (log-uniform, as the survey)
+MIN_CELLS_PER_POINT = 20
+POINT_LEVELS = 12
 REPS = 5
 MARGIN = 0.03        # a tile must beat the default rule by this fraction
 
@@ -86,20 +88,31 @@ class BucketBench:
         self.n1, self.p2, self.nf = n1, p2, nf
         n = n1 << p2
         self.ncurves = int(np.clip(round(CELLS / n), 4, 50_000))
-        self.npts = int(np.clip(round(DENSITY * n), 50, 20_000))
         rng = np.random.default_rng(seed)
         T = 100.0
         df = 1.0 / (5.0 * T)
         f0 = 20.0 * df   # f (t_max - t_min) > 3 everywhere: no low-frequency direct sums
-        t = np.sort(rng.uniform(0.0, T, self.npts))
-        y = np.sin(2.0 * np.pi * t / 0.7) + rng.normal(0.0, 0.3, self.npts)
-        dy = 0.3 + 0.2 * rng.uniform(size=self.npts)
-        self.curves = [_lib.gls_curve(obs_offset=i * self.npts, n_obs=self.npts, nf=nf, f0=f0,
-                                      df=df, out_offset=i * nf, ybar=0.0, yy=1.0)
-                       for i in range(self.ncurves)]
-        self.d_t = DeviceArray.to_device(np.tile(t, self.ncurves))
-        self.d_a = DeviceArray.to_device(np.tile(y, self.ncurves))
-        self.d_b = DeviceArray.to_device(np.tile(dy, self.ncurves))
+        # points per curve as in the survey workload: log-uniform over
+        # [POINTS_MIN, POINTS_MAX] (at most one point per MIN_CELLS_PER_POINT
+        # cells), drawn from POINT_LEVELS levels; each curve takes an evenly
+        # spread subset of one sorted pool of times over [0, T)
+        cap = max(POINTS_MIN, min(POINTS_MAX, n // MIN_CELLS_PER_POINT))
+        levels = np.unique(np.geomspace(POINTS_MIN, cap, POINT_LEVELS).astype(int))
+        pick = rng.integers(0, len(levels), size=self.ncurves)
+        pool = np.sort(rng.uniform(0.0, T, int(levels[-1])))
+        idx = {int(v): np.linspace(0, len(pool) - 1, int(v)).astype(np.int64) for v in levels}
+        counts = levels[pick]
+        t = np.concatenate([pool[idx[int(v)]] for v in counts])
+        y = np.sin(2.0 * np.pi * t / 0.7) + rng.normal(0.0, 0.3, t.size)
+        dy = 0.3 + 0.2 * rng.uniform(size=t.size)
+        offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        self.npts = float(np.mean(counts))
+        self.curves = [_lib.gls_curve(obs_offset=int(o), n_obs=int(c), nf=nf, f0=f0, df=df,
+                                      out_offset=i * nf, ybar=0.0, yy=1.0)
+                       for i, (o, c) in enumerate(zip(offs, counts))]
+        self.d_t = DeviceArray.to_device(t)
+        self.d_a = DeviceArray.to_device(y)
+        self.d_b = DeviceArray.to_device(dy)
         self.d_power = DeviceArray(self.ncurves * nf, np.float32)
         self.d_ws = None
 
@@ -194,7 +207,7 @@ def tune_bucket(n1, p2, nf, reps=REPS, log=print):
            "pass1_ms": {str(k): round(v, 4) for k, v in t1.items()},
            "pass2_ms": {str(k): round(v, 4) for k, v in t2.items()},
            "max_power_diff_between_tiles": dmax}
-    log(f"bucket n1={n1} p2={p2} ({bb.ncurves} curves x {bb.npts} points, nf={nf}): "
+    log(f"bucket n1={n1} p2={p2} ({bb.ncurves} curves x {bb.npts:.0f} points, nf={nf}): "
         f"c1={c1} {rec['pass1_ms']}  cr={cr} {rec['pass2_ms']}  "
         f"{rec['ps_per_cell']} ps/cell", flush=True)
     return rec
@@ -218,7 +231,8 @@ def main(argv=None):
         keys = keys[:args.max_buckets]
     rows = [tune_bucket(n1, p2, buckets[(n1, p2)], args.reps) for n1, p2 in keys]
     doc = {"arch": "gfx950", "sigma": SIGMA, "m": M, "window": "exponential of semicircle",
-           "cells_per_grid": CELLS, "points_per_cell": DENSITY, "reps": args.reps,
+           "cells_per_grid": CELLS, "points_per_curve": [POINTS_MIN, POINTS_MAX],
+           "min_cells_per_point": MIN_CELLS_PER_POINT, "reps": args.reps,
            "margin": MARGIN, "seconds": round(time.time() - t0, 1), "buckets": rows}
     with open(args.out, "w") as f:
         json.dump(doc, f, indent=1)
