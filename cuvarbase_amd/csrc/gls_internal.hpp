@@ -111,12 +111,14 @@ struct CurveState {
     double tmean;      // mean of the finite t (FP32-coordinate plans only)
 };
 
-// per-point workspace (structure of arrays, indexed by CurveGeom::pts + j)
+// per-point workspace, indexed by CurveGeom::pts + j.  A node record holds a
+// point's first cell U, the bits of its fractional offset fr in [0, 1) and of
+// its modulated value (re, im) on one grid, so that pass 1 fetches a point
+// with one 16-byte load.
 struct PointArrays {
-    int32_t* U1;   float* fr1;     // node set 1 (SYW, W1): sorted by bucket
-    float2* v0;    float2* v1;     // modulated yw, w
-    int32_t* U2;   float* fr2;     // node set 2 (W2 at 2f)
-    float2* v2;                    // modulated w at 2f
+    int4* nd0;     // grid 0 (yw at f): node set 1, in bucket order
+    int4* nd1;     // grid 1 (w at f): node set 1 (same U, fr as nd0)
+    int4* nd2;     // grid 2 (w at 2f): node set 2
     float* w;      float* yw;      // prepared weights (direct method)
     double* dt;                    // t - tref (direct method)
 };

@@ -82,29 +82,19 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
     }
     const int len0 = e0 - s0;
     const int total = len0 + (e1 - s1);
-    const int32_t* Ua = (g == 1) ? pa.U1 : pa.U2;
-    const float* Fa = (g == 1) ? pa.fr1 : pa.fr2;
-    const float2* Va = (gi == 0) ? pa.v0 : (gi == 1 ? pa.v1 : pa.v2);
+    // node records {U, fr, value} of this grid, one 16-byte load per point
+    const int4* Na = (gi == 0) ? pa.nd0 : (gi == 1 ? pa.nd1 : pa.nd2);
 
     // the first PF points of each thread are loaded before the tile is
     // cleared and the twiddle tables are built, so their latency overlaps
     // that work
     constexpr int PF = 2;
-    int pU[PF];
-    float pfr[PF];
-    float2 pv[PF];
+    int4 pn[PF];
 #pragma unroll
     for (int f = 0; f < PF; ++f) {
         const int q = tid + f * NT1;
-        pU[f] = 0;
-        pfr[f] = 0.f;
-        pv[f] = make_float2(0.f, 0.f);
-        if (q < total) {
-            const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
-            pU[f] = Ua[o];
-            pfr[f] = Fa[o];
-            pv[f] = Va[o];
-        }
+        pn[f] = make_int4(0, 0, 0, 0);
+        if (q < total) pn[f] = Na[G.pts + (q < len0 ? s0 + q : s1 + (q - len0))];
     }
 
     for (int i = tid; i < TL / 2; i += NT1)
@@ -141,18 +131,17 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
         // largest |component| as non-negative float bits (next N1 ints)
         int* cnt = reinterpret_cast<int*>(tile);
         int* mxb = cnt + N1;
-        auto count = [&](int U, float2 v) {
-            const int r = row_of(U);
+        auto count = [&](int4 nd) {
+            const int r = row_of(nd.x);
             atomicAdd(&cnt[r], 1);
-            atomicMax(&mxb[r], __float_as_int(fmaxf(fabsf(v.x), fabsf(v.y))));
+            atomicMax(&mxb[r], __float_as_int(fmaxf(fabsf(__int_as_float(nd.z)),
+                                                    fabsf(__int_as_float(nd.w)))));
         };
 #pragma unroll
         for (int f = 0; f < PF; ++f)
-            if (tid + f * NT1 < total) count(pU[f], pv[f]);
-        for (int q = tid + PF * NT1; q < total; q += NT1) {
-            const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
-            count(Ua[o], Va[o]);
-        }
+            if (tid + f * NT1 < total) count(pn[f]);
+        for (int q = tid + PF * NT1; q < total; q += NT1)
+            count(Na[G.pts + (q < len0 ? s0 + q : s1 + (q - len0))]);
         __syncthreads();
         for (int r = tid; r < N1; r += NT1)
             rexp[r] = (signed char)spread_exponent((float)cnt[r] * __int_as_float(mxb[r]));
@@ -161,7 +150,10 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
             tile[i] = make_float2(0.f, 0.f);
         __syncthreads();
     }
-    auto spread = [&](int U, float fr, float2 v) {
+    auto spread = [&](int4 nd) {
+        const int U = nd.x;
+        const float fr = __int_as_float(nd.y);
+        float2 v = make_float2(__int_as_float(nd.z), __int_as_float(nd.w));
         if (FIX) {
             const float sc = ldexpf(1.0f, rexp[row_of(U)]);
             v = make_float2(v.x * sc, v.y * sc);
@@ -197,11 +189,9 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
     };
 #pragma unroll
     for (int f = 0; f < PF; ++f)
-        if (tid + f * NT1 < total) spread(pU[f], pfr[f], pv[f]);
-    for (int q = tid + PF * NT1; q < total; q += NT1) {
-        const int64_t o = G.pts + (q < len0 ? s0 + q : s1 + (q - len0));
-        spread(Ua[o], Fa[o], Va[o]);
-    }
+        if (tid + f * NT1 < total) spread(pn[f]);
+    for (int q = tid + PF * NT1; q < total; q += NT1)
+        spread(Na[G.pts + (q < len0 ? s0 + q : s1 + (q - len0))]);
     __syncthreads();
     if (FIX) {
         // integer cells -> float2 with their row's scale (exact powers of two)

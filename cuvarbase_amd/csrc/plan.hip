@@ -49,8 +49,8 @@ struct gls_plan {
     size_t off_geom = 0, off_bgeom = 0, off_tiles = 0, off_prep = 0;
     std::vector<int32_t> prep_list;   // curves prepared by 256-thread blocks, then wide ones
     int nprep_small = 0;
-    size_t off_state = 0, off_U1 = 0, off_fr1 = 0, off_v0 = 0, off_v1 = 0, off_U2 = 0,
-           off_fr2 = 0, off_v2 = 0, off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
+    size_t off_state = 0, off_nd0 = 0, off_nd1 = 0, off_nd2 = 0,
+           off_w = 0, off_yw = 0, off_dt = 0, off_bkt = 0,
            off_grid = 0;
     // per wide curve bucketing records (NFFT method; see WideAux) and chunk
     // partials of their reductions (WidePart)
@@ -301,14 +301,10 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         P->buckets.push_back(B);
     }
     const int64_t grid_elems = (chunk > 0) ? grid_region : grid_total;
-    P->off_U1 = L.take(sizeof(int32_t) * total_pts);
-    P->off_fr1 = L.take(sizeof(float) * total_pts);
-    P->off_v0 = L.take(sizeof(float2) * total_pts);
+    P->off_nd0 = L.take(sizeof(int4) * total_pts);
     if (ngrids == 3) {
-        P->off_v1 = L.take(sizeof(float2) * total_pts);
-        P->off_U2 = L.take(sizeof(int32_t) * total_pts);
-        P->off_fr2 = L.take(sizeof(float) * total_pts);
-        P->off_v2 = L.take(sizeof(float2) * total_pts);
+        P->off_nd1 = L.take(sizeof(int4) * total_pts);
+        P->off_nd2 = L.take(sizeof(int4) * total_pts);
         // prepared points for the low-frequency direct sums
         P->off_w = L.take(sizeof(float) * total_pts);
         P->off_yw = L.take(sizeof(float) * total_pts);
@@ -546,13 +542,9 @@ void destroy_plan(gls_plan* P) {
 PointArrays point_arrays(const gls_plan* P, void* ws) {
     char* base = (char*)ws;
     PointArrays pa{};
-    pa.U1 = (int32_t*)(base + P->off_U1);
-    pa.fr1 = (float*)(base + P->off_fr1);
-    pa.v0 = (float2*)(base + P->off_v0);
-    pa.v1 = (float2*)(base + P->off_v1);
-    pa.U2 = (int32_t*)(base + P->off_U2);
-    pa.fr2 = (float*)(base + P->off_fr2);
-    pa.v2 = (float2*)(base + P->off_v2);
+    pa.nd0 = (int4*)(base + P->off_nd0);
+    pa.nd1 = (int4*)(base + P->off_nd1);
+    pa.nd2 = (int4*)(base + P->off_nd2);
     // offset 0 is the descriptor region: it marks an array the plan lacks
     pa.w = P->off_w ? (float*)(base + P->off_w) : nullptr;
     pa.yw = P->off_yw ? (float*)(base + P->off_yw) : nullptr;
@@ -786,8 +778,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         P->off_state = L.take(sizeof(CurveState) * (size_t)ncurves);
         // re-take the data regions after the meta and state regions
         const size_t shift = L.cur;
-        for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_v1, &P->off_U2,
-                          &P->off_fr2, &P->off_v2, &P->off_w, &P->off_yw, &P->off_dt,
+        for (size_t* o : {&P->off_nd0, &P->off_nd1, &P->off_nd2, &P->off_w, &P->off_yw,
+                          &P->off_dt,
                           &P->off_bkt, &P->off_wide, &P->off_wpart, &P->off_grid})
             *o += shift;
         L.cur += probe.cur;
@@ -859,7 +851,7 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     reserve_meta(P, L, bcurves.size(), 0);
     P->off_state = L.take(sizeof(CurveState) * (size_t)nitems);
     const size_t shift = L.cur;
-    for (size_t* o : {&P->off_U1, &P->off_fr1, &P->off_v0, &P->off_bkt, &P->off_wide,
+    for (size_t* o : {&P->off_nd0, &P->off_bkt, &P->off_wide,
                       &P->off_wpart, &P->off_grid})
         *o += shift;
     L.cur += probe.cur;

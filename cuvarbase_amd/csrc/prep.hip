@@ -132,15 +132,15 @@ __device__ __forceinline__ void prep_emit(const PointArrays& pa, const CurveGeom
         const float2 e = prep_phase<F32>(dt, g, Fc);
         const int pos = cursor(g - 1, (U & (n2 - 1)) / G.c1);
         const int64_t o = G.pts + pos;
+        auto rec = [&](double s) {
+            return make_int4(U, __float_as_int(fr), __float_as_int((float)s * e.x),
+                             __float_as_int((float)s * e.y));
+        };
         if (g == 1) {
-            pa.U1[o] = U;
-            pa.fr1[o] = fr;
-            pa.v0[o] = make_float2((float)yw * e.x, (float)yw * e.y);
-            if (KIND != 2) pa.v1[o] = make_float2((float)w * e.x, (float)w * e.y);
+            pa.nd0[o] = rec(yw);
+            if (KIND != 2) pa.nd1[o] = rec(w);
         } else {
-            pa.U2[o] = U;
-            pa.fr2[o] = fr;
-            pa.v2[o] = make_float2((float)w * e.x, (float)w * e.y);
+            pa.nd2[o] = rec(w);
         }
     }
 }
