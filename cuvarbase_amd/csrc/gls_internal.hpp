@@ -200,7 +200,8 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        hipStream_t s);
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
-// (c1: the bucket's pass-1 tile, cr: its pass-2 tile)
+// (c1: the bucket's pass-1 tile, cr: its pass-2 tile); pass 1's bkt points at
+// the first curve's bucket offsets, which the others' follow, 2 (nb + 1) each
 hipError_t launch_pass1(int n1, int c1, const CurveGeom* bgeom, int ncurves, int ngrids,
                         int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s);

@@ -60,13 +60,16 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
     const int bid = xcd_logical_block(blockIdx.x, gridDim.x);
     const int jb = bid % nb;
     const int gi = (bid / nb) % ngrids;
-    const CurveGeom& G = bgeom[bid / (nb * ngrids)];
+    const int ce = bid / (nb * ngrids);   // the curve's place in the bucket entry
+    const CurveGeom& G = bgeom[ce];
     const int n = G.n;
     const int a = jb * C1;
 
-    // point set: buckets jb-nbk .. jb (cyclic), at most two contiguous ranges
+    // point set: buckets jb-nbk .. jb (cyclic), at most two contiguous ranges.
+    // The entry's curves keep their bucket offsets in order, 2 (nb + 1) each,
+    // so these loads need not wait for the curve's record
     const int g = (gi == 2) ? 2 : 1;
-    const int32_t* bk = bkt + G.bkt + (int64_t)(g - 1) * (nb + 1);
+    const int32_t* bk = bkt + (int64_t)ce * (2 * (nb + 1)) + (int64_t)(g - 1) * (nb + 1);
     const int nbk = (2 * m + C1 - 1) / C1;
     int s0, e0, s1, e1;
     const int first = jb - nbk;

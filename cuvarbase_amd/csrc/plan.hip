@@ -35,6 +35,7 @@ struct gls_plan {
         int n1 = 0, p2 = 0, c1 = 0, cr = 0;   // size and tiles (columns, rows)
         int64_t first = 0;  // offset (curves) into the bucket curve list
         int count = 0;
+        int64_t bkt0 = 0;   // bucket offsets of its first curve (the others follow)
     };
     std::vector<Bucket> buckets;   // one entry per chunk of a size bucket
     std::vector<int> bucket_lane;  // 0: the caller's stream, j > 0: auxiliary stream j-1
@@ -266,8 +267,6 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         G.g_norm = (float)(1.0 / std::sqrt(M_PI * G.b));
         G.g_bpi2 = (float)(G.b * M_PI * M_PI);
         G.ninv = 1.0f / (float)n;
-        G.bkt = bkt_ints;
-        bkt_ints += 2 * (n2 / c1 + 1);
         by_size[{n1, p2}].push_back(i);
     }
     const int64_t chunk = P->chunk_bytes;
@@ -281,6 +280,7 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         B.cr = bt.cr;
         B.first = (int64_t)bcurves.size();
         B.count = 0;
+        B.bkt0 = bkt_ints;
         int64_t acc = 0;   // grid elements of the open entry
         for (int32_t ci : kv.second) {
             CurveGeom& G = P->geom[ci];
@@ -289,8 +289,13 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
                 P->buckets.push_back(B);
                 B.first = (int64_t)bcurves.size();
                 B.count = 0;
+                B.bkt0 = bkt_ints;
                 acc = 0;
             }
+            // bucket offsets in entry order, 2 (nb + 1) per curve, so that
+            // pass 1 finds a curve's from its place in the entry alone
+            G.bkt = bkt_ints;
+            bkt_ints += 2 * ((1 << G.p2) / G.c1 + 1);
             G.grid = (chunk > 0) ? acc : grid_total;
             acc += cells;
             grid_total += cells;
@@ -603,8 +608,8 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             const auto& B = P->buckets[bi];
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
             const CurveGeom* bgeom = d_bgeom + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, bgeom, B.count, ngrids, B.p2, pa, bkt, grids,
-                                     pp, ls));
+            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, bgeom, B.count, ngrids, B.p2, pa, bkt + B.bkt0,
+                                     grids, pp, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
                 GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
