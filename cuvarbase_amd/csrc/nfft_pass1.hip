@@ -29,8 +29,9 @@ constexpr int pass1_threads(int n1, int c1) { return n1 * c1 >= 6144 ? 512 : 256
 // pass 1: spread + column FFT + inter-pass twiddle
 // ---------------------------------------------------------------------------
 // FIX: fixed-point spreading (GLS_SPREAD_FIXED): the tile holds int32 pairs
-// while the taps are added, each tap rounded to a multiple of 2^-e, then is
-// converted to float2 in place before the FFT.  All the cells a point touches
+// while the taps are added, each tap rounded to a multiple of 2^-e and added
+// as one 64-bit integer im 2^32 + re, then is converted to float2 in place
+// before the FFT.  All the cells a point touches
 // in a tile lie in one row (its columns are consecutive and do not wrap), so
 // e is chosen per row: from the number of the tile's points in the row times
 // their largest |component| (window weights are at most 1), no cell of the
@@ -180,9 +181,13 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
                 }
                 float2* cellp = &tile[lds_at<BS>(u1, jj)];
                 if (FIX) {
-                    int* ip = reinterpret_cast<int*>(cellp);
-                    atomicAdd(ip, __float2int_rn(v.x * wgt));
-                    atomicAdd(ip + 1, __float2int_rn(v.y * wgt));
+                    // both components in one 64-bit integer add: the cell
+                    // holds im 2^32 + re (two's complement), exact while
+                    // |re|, |im| < 2^31, which the row exponent ensures
+                    const long long re = __float2int_rn(v.x * wgt);
+                    const long long im = __float2int_rn(v.y * wgt);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(cellp),
+                              (unsigned long long)(im * 4294967296LL + re));
                 } else {
                     atomicAdd(&cellp->x, v.x * wgt);
                     atomicAdd(&cellp->y, v.y * wgt);
@@ -197,13 +202,16 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
         spread(Na[G.pts + (q < len0 ? s0 + q : s1 + (q - len0))]);
     __syncthreads();
     if (FIX) {
-        // integer cells -> float2 with their row's scale (exact powers of two)
+        // integer cells -> float2 with their row's scale (exact powers of
+        // two).  A cell's 64-bit sum im 2^32 + re keeps re in its low word
+        // and im - 1 in its high word when re < 0
         constexpr int LC = ilog2_c(C1);
         for (int i = tid; i < N1 * C1; i += NT1) {
             float2* cp = &tile[i + (i >> 4)];   // lds_at<BS>(i / C1, i % C1)
             const int2 c = *reinterpret_cast<const int2*>(cp);
+            const int im = c.y + (c.x < 0 ? 1 : 0);
             const float inv = ldexpf(1.0f, -(int)rexp[i >> LC]);
-            *cp = make_float2((float)c.x * inv, (float)c.y * inv);
+            *cp = make_float2((float)c.x * inv, (float)im * inv);
         }
         __syncthreads();
     }
