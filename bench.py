@@ -270,9 +270,10 @@ def main():
                     help="extra steps timed with concurrent size buckets (0: skip)")
     ap.add_argument("--e2e", type=int, default=1,
                     help="time the end-to-end (host memory to host memory) batch (0: skip)")
-    ap.add_argument("--spread", type=int, default=0,
+    ap.add_argument("--spread", type=int, default=2, choices=(0, 1, 2),
                     help="batched path's pass-1 accumulation: 0 FP32 atomics, 1 fixed point "
-                         "(bitwise reproducible); the server always runs fixed point")
+                         "(bitwise reproducible), 2 per curve by point density (the "
+                         "default); the server always runs fixed point")
     ap.add_argument("--grid-chunk-mib", type=int, default=0,
                     help="NFFT grid chunk size in MiB (0: no chunking)")
     ap.add_argument("--calib-copy-mib", type=int, default=2048,
@@ -302,7 +303,7 @@ def main():
                                                                  keep=args.parity_sample)
     long_P = None if (args.no_cpu or not args.shapes) else long_curve_oracle()
 
-    from cuvarbase_amd import engine, _device
+    from cuvarbase_amd import engine, _device, _lib
     from cuvarbase_amd.lombscargle import LombScargleAsyncProcess
 
     chunk = args.grid_chunk_mib << 20 if args.grid_chunk_mib > 0 else 0
@@ -341,8 +342,8 @@ def main():
     for _ in range(n_instr):
         batch.execute(stream=s)
         s.synchronize()
-        for (name, p, ms, by) in batch.plan.stage_times():
-            a = stage_acc.setdefault((name, p), [0, 0.0, 0.0])
+        for i, (name, p, ms, by) in enumerate(batch.plan.stage_times()):
+            a = stage_acc.setdefault((i, name), [0, 0.0, 0.0])
             a[0] += 1
             a[1] += ms
             a[2] += by
@@ -369,18 +370,19 @@ def main():
     ncurves = batch.ncurves
     ws_bytes = batch.plan.workspace_bytes
     geo = [batch.plan.geometry(i) for i in range(ncurves)]
-    tiles = [batch.plan.tiles(i) for i in range(ncurves)]
+    n_fixed = sum(batch.plan.spread(i) == _lib.SPREAD_FIXED for i in range(ncurves))
 
-    # dominant kernel: the kernel template (pass-1 column length n1 and tile
-    # c1, or pass-2 row length n2 and tile cr) with the most time over the
-    # timed steps
-    split = {g[0]: (g[1], g[2], tl[0], tl[1]) for g, tl in zip(geo, tiles)}
+    # dominant kernel: the kernel template (pass-1 column length n1, tile c1
+    # and accumulation, or pass-2 row length n2 and tile cr) with the most
+    # time over the instrumented steps
     by_kernel, by_kind = {}, {}
-    for (name, p), (c, ms, by) in stage_acc.items():
+    for (i, name), (c, ms, by) in stage_acc.items():
+        n1, p2, c1, cr, sp = batch.plan.stage_bucket(i)
         if name == "pass1":
-            key = f"pass1_kernel<{split[p][0]}, {split[p][2]}, {'true' if args.spread else 'false'}>"
+            fx = "true" if sp == _lib.SPREAD_FIXED else "false"
+            key = f"pass1_kernel<{n1}, {c1}, {fx}>"
         elif name == "pass2":
-            key = f"pass2_kernel<{int(math.log2(split[p][1]))}, {split[p][3]}, 0>"
+            key = f"pass2_kernel<{p2}, {cr}, 0>"
         else:
             key = name
         for d, k in ((by_kernel, key), (by_kind, name)):
@@ -596,7 +598,9 @@ def main():
             "window_beta": cfg["window_beta"],
             "grid_cells_per_step": int(3 * sum(g[0] for g in geo)),
             "grid_chunk_bytes": int(chunk),
-            "spread": "fixed point" if args.spread else "fp32 atomics",
+            "spread": {0: "fp32 atomics", 1: "fixed point",
+                       2: f"per curve: fixed point for {n_fixed} curves of at least one "
+                          "point per 80 cells, fp32 atomics for the others"}[args.spread],
             "workspace_bytes": int(ws_bytes),
         },
         "roofline": {

@@ -39,6 +39,7 @@ PRECISION_FP64_COORDS = 0
 PRECISION_FP32 = 1
 SPREAD_FLOAT = 0
 SPREAD_FIXED = 1
+SPREAD_AUTO = 2
 
 STATIC_SCRATCH_BYTES = 128 << 20   # GLS_STATIC_SCRATCH_BYTES
 
@@ -122,6 +123,8 @@ _PROTOS = {
     "gls_plan_curve_geometry": [_vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "gls_plan_curve_tiles": [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "gls_plan_curve_spread": [_vp, C.c_int32, C.POINTER(C.c_int32)],
+    "gls_plan_stage_bucket": [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 5,
     "gls_tile_set": [C.c_int32, C.c_int32, C.c_int32, C.c_int32],
     "gls_tile_get": [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "gls_grid_cost_set": [C.c_int32, C.c_int32, C.c_double],

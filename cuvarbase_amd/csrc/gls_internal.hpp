@@ -134,7 +134,6 @@ struct PlanParams {
     int32_t window;    // GLS_WINDOW_*
     float es_beta;     // ES shape parameter
     int32_t fp32_coords;  // GLS_PRECISION_FP32: FP32 time offsets, cells and phases
-    int32_t spread_fixed; // GLS_SPREAD_FIXED: integer (fixed-point) LDS accumulation
     // ES: 1/psi_hat(xi) at xi = i / (2 kEsTab), i = 0 .. kEsTab + 1 (the
     // library's resident device copy for (m, beta), es_table_device)
     const float* es_deconv;
@@ -198,12 +197,16 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
                        int y_is_f32, WideAux* aux, WidePart* part, int wide_max_obs,
                        hipStream_t s);
+// GLS_SPREAD_AUTO spreads a curve in fixed point when it has at least one
+// point per this many grid cells
+constexpr int kFixedCellsPerPoint = 80;
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
-// (c1: the bucket's pass-1 tile, cr: its pass-2 tile); pass 1's bkt points at
-// the first curve's bucket offsets, which the others' follow, 2 (nb + 1) each
-hipError_t launch_pass1(int n1, int c1, const CurveGeom* bgeom, int ncurves, int ngrids,
-                        int p2, PointArrays pa, const int32_t* bkt, float2* grids,
+// (c1: the bucket's pass-1 tile, cr: its pass-2 tile; fix: fixed-point
+// spreading); pass 1's bkt points at the first curve's bucket offsets, which
+// the others' follow, 2 (nb + 1) each
+hipError_t launch_pass1(int n1, int c1, bool fix, const CurveGeom* bgeom, int ncurves,
+                        int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s);
 hipError_t launch_pass2_lomb(int p2, int cr, int n1, const CurveGeom* bgeom,
                              const CurveState* state, int ncurves, const float2* grids,

@@ -249,14 +249,14 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
 }
 
 template <int N1, int C1>
-hipError_t pass1_dispatch(const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
+hipError_t pass1_dispatch(bool fix, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
                           PointArrays pa, const int32_t* bkt, float2* grids,
                           const PlanParams& pp, hipStream_t s) {
     const int nb = (1 << p2) / C1;
     const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
     if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
     const float ainv = 1.0f / ((float)pp.m + 0.5f);
-    if (pp.spread_fixed)
+    if (fix)
         hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), dim3((unsigned)nblocks),
                            dim3(pass1_threads(N1, C1)), 0,
                            s, bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
@@ -280,13 +280,13 @@ bool pass1_tile_built(int n1, int c1) {
 }
 
 // the (n1, c1) pairs this library has kernels for
-hipError_t launch_pass1(int n1, int c1, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
-                        PointArrays pa, const int32_t* bkt, float2* grids,
+hipError_t launch_pass1(int n1, int c1, bool fix, const CurveGeom* bgeom, int ncurves,
+                        int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
 #define GLS_P1_CASE(N, C)  \
     if (n1 == N && c1 == C) \
-        return pass1_dispatch<N, C>(bgeom, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
+        return pass1_dispatch<N, C>(fix, bgeom, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
     GLS_PASS1_TILES(GLS_P1_CASE)
 #undef GLS_P1_CASE
     return hipErrorInvalidValue;

@@ -20,6 +20,7 @@
 #include <atomic>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "gls_internal.hpp"
 
@@ -36,8 +37,11 @@ struct gls_plan {
         int64_t first = 0;  // offset (curves) into the bucket curve list
         int count = 0;
         int64_t bkt0 = 0;   // bucket offsets of its first curve (the others follow)
+        bool fix = false;   // fixed-point spreading in pass 1
     };
-    std::vector<Bucket> buckets;   // one entry per chunk of a size bucket
+    std::vector<Bucket> buckets;   // one entry per chunk of a size bucket and spread mode
+    int spread_mode = GLS_SPREAD_FLOAT;  // gls_params.spread
+    std::vector<int8_t> curve_fixed;     // per curve: pass 1 spreads it in fixed point
     std::vector<int> bucket_lane;  // 0: the caller's stream, j > 0: auxiliary stream j-1
     bool fork_buckets = false;     // run the lanes concurrently (gls_plan_set_fork)
     std::vector<hipEvent_t> fork_join;  // fork event + one join event per aux stream
@@ -65,6 +69,7 @@ struct gls_plan {
         int kind;      // 0 prep, 1 pass 1, 2 pass 2, 3 direct sums
         int grid;      // grid size n of the bucket (0 for prep / direct)
         double bytes;  // algorithmic bytes moved by the stage
+        int bucket;    // index into buckets (passes 1 and 2), else -1
     };
     std::vector<Stage> stages;
     bool timing = false;
@@ -244,7 +249,9 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
     const int m = P->pp.m;
     TileCache tiles;
     const GridCosts costs;
-    std::map<std::pair<int, int>, std::vector<int32_t>> by_size;
+    // (n1, p2, fixed-point spreading) -> curves
+    std::map<std::tuple<int, int, int>, std::vector<int32_t>> by_size;
+    P->curve_fixed.assign(P->ncurves, 0);
     int64_t bkt_ints = 0;
     for (int i = 0; i < P->ncurves; ++i) {
         CurveGeom& G = P->geom[i];
@@ -267,14 +274,22 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         G.g_norm = (float)(1.0 / std::sqrt(M_PI * G.b));
         G.g_bpi2 = (float)(G.b * M_PI * M_PI);
         G.ninv = 1.0f / (float)n;
-        by_size[{n1, p2}].push_back(i);
+        // GLS_SPREAD_AUTO: fixed point for curves of at least one point per
+        // kFixedCellsPerPoint grid cells, where its integer adds outrun the
+        // FP32 atomics of dense tiles
+        const int fix = P->spread_mode == GLS_SPREAD_FIXED ||
+                        (P->spread_mode == GLS_SPREAD_AUTO &&
+                         (int64_t)G.n_obs * kFixedCellsPerPoint >= n);
+        P->curve_fixed[i] = (int8_t)fix;
+        by_size[{n1, p2, fix}].push_back(i);
     }
     const int64_t chunk = P->chunk_bytes;
     int64_t grid_total = 0, grid_region = 0;
     for (auto& kv : by_size) {
         gls_plan::Bucket B;
-        B.n1 = kv.first.first;
-        B.p2 = kv.first.second;
+        B.n1 = std::get<0>(kv.first);
+        B.p2 = std::get<1>(kv.first);
+        B.fix = std::get<2>(kv.first) != 0;
         const BucketTiles bt = tiles.get(B.n1, B.p2);
         B.c1 = bt.c1;
         B.cr = bt.cr;
@@ -466,12 +481,13 @@ void build_stage_list(gls_plan* P) {
     double prep = 0.0;
     for (auto& G : P->geom) {
         const double in_bytes = (P->pp.input_kind == GLS_INPUT_RAW || P->kind == 1) ? 24.0 : 16.0;
-        const double rec = (P->pp.method == GLS_METHOD_NFFT) ? (ngr == 3 ? 40.0 : 16.0) : 16.0;
+        const double rec = (P->pp.method == GLS_METHOD_NFFT) ? 16.0 * ngr : 16.0;
         prep += (double)G.n_obs * (in_bytes + rec);
     }
-    P->stages.push_back({0, 0, prep});
+    P->stages.push_back({0, 0, prep, -1});
     if (P->pp.method == GLS_METHOD_NFFT) {
-        for (const auto& B : P->buckets) {
+        for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
+            const auto& B = P->buckets[bi];
             double b1 = 0.0, b2 = 0.0;
             for (int q = 0; q < B.count; ++q) {
                 const CurveGeom& G = P->geom[P->bcurves[B.first + q]];
@@ -479,18 +495,18 @@ void build_stage_list(gls_plan* P) {
                 b1 += ngr * (16.0 * G.n_obs + 8.0 * n);
                 b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : 4.0) * G.nf;
             }
-            P->stages.push_back({1, B.n1 << B.p2, b1});
-            P->stages.push_back({2, B.n1 << B.p2, b2});
+            P->stages.push_back({1, B.n1 << B.p2, b1, (int)bi});
+            P->stages.push_back({2, B.n1 << B.p2, b2, (int)bi});
         }
         if (P->kind == 0 && !P->pp.fp32_coords) {
             double bl = 0.0;
             for (auto& G : P->geom) bl += 16.0 * G.n_obs;
-            P->stages.push_back({4, 0, bl});
+            P->stages.push_back({4, 0, bl, -1});
         }
     } else {
         double b = 0.0;
         for (auto& G : P->geom) b += 16.0 * G.n_obs + 4.0 * G.nf;
-        P->stages.push_back({3, 0, b});
+        P->stages.push_back({3, 0, b, -1});
     }
 }
 
@@ -608,8 +624,8 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             const auto& B = P->buckets[bi];
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
             const CurveGeom* bgeom = d_bgeom + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, bgeom, B.count, ngrids, B.p2, pa, bkt + B.bkt0,
-                                     grids, pp, ls));
+            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, B.fix, bgeom, B.count, ngrids, B.p2, pa,
+                                     bkt + B.bkt0, grids, pp, ls));
             GLS_HIP_TRY(mark());
             if (P->kind == 1)
                 GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
@@ -708,7 +724,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
         return fail(GLS_E_INVALID, "unknown precision");
     if (prm.precision == GLS_PRECISION_FP32 && prm.method != GLS_METHOD_NFFT)
         return fail(GLS_E_INVALID, "GLS_PRECISION_FP32 is an NFFT-method option");
-    if (prm.spread != GLS_SPREAD_FLOAT && prm.spread != GLS_SPREAD_FIXED)
+    if (prm.spread != GLS_SPREAD_FLOAT && prm.spread != GLS_SPREAD_FIXED &&
+        prm.spread != GLS_SPREAD_AUTO)
         return fail(GLS_E_INVALID, "unknown spread mode");
     gls_plan* P = new gls_plan();
     P->id = g_plan_ids.fetch_add(1);
@@ -722,7 +739,7 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     P->pp.kind = 0;
     P->pp.window = (prm.method == GLS_METHOD_NFFT) ? prm.window : GLS_WINDOW_GAUSSIAN;
     P->pp.fp32_coords = prm.precision == GLS_PRECISION_FP32 ? 1 : 0;
-    P->pp.spread_fixed = prm.spread == GLS_SPREAD_FIXED ? 1 : 0;
+    P->spread_mode = prm.spread;
     P->pp.es_beta = prm.window_beta > 0.f
                         ? prm.window_beta
                         : (float)(M_PI * (2 * prm.m + 1) * (1.0 - 0.5 / prm.sigma));
@@ -914,13 +931,23 @@ int gls_plan_curve_tiles(const gls_plan* plan, int32_t i, int32_t* c1, int32_t* 
     int tc = 0, tr = 0;
     if (plan->pp.method == GLS_METHOD_NFFT)
         for (const auto& B : plan->buckets)
-            if (B.n1 == G.n1 && B.p2 == G.p2) {
+            if (B.n1 == G.n1 && B.p2 == G.p2 && B.fix == (plan->curve_fixed[i] != 0)) {
                 tc = B.c1;
                 tr = B.cr;
                 break;
             }
     if (c1) *c1 = tc;
     if (cr) *cr = tr;
+    return GLS_OK;
+}
+
+int gls_plan_curve_spread(const gls_plan* plan, int32_t i, int32_t* spread) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (i < 0 || i >= plan->ncurves) return fail(GLS_E_INVALID, "curve index out of range");
+    if (!spread) return fail(GLS_E_INVALID, "NULL output");
+    const bool fix = plan->pp.method == GLS_METHOD_NFFT && i < (int)plan->curve_fixed.size() &&
+                     plan->curve_fixed[i] != 0;
+    *spread = fix ? GLS_SPREAD_FIXED : GLS_SPREAD_FLOAT;
     return GLS_OK;
 }
 
@@ -1069,6 +1096,21 @@ int gls_plan_enable_timing(gls_plan* plan, int32_t enable) {
 int gls_plan_stage_count(const gls_plan* plan, int32_t* count) {
     if (!plan || !count) return fail(GLS_E_INVALID, "NULL argument");
     *count = (int32_t)plan->stages.size();
+    return GLS_OK;
+}
+
+int gls_plan_stage_bucket(const gls_plan* plan, int32_t stage, int32_t* n1, int32_t* p2,
+                          int32_t* c1, int32_t* cr, int32_t* spread) {
+    if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
+    if (stage < 0 || stage >= (int)plan->stages.size())
+        return fail(GLS_E_INVALID, "stage index out of range");
+    const int b = plan->stages[stage].bucket;
+    const gls_plan::Bucket* B = b >= 0 ? &plan->buckets[b] : nullptr;
+    if (n1) *n1 = B ? B->n1 : 0;
+    if (p2) *p2 = B ? B->p2 : 0;
+    if (c1) *c1 = B ? B->c1 : 0;
+    if (cr) *cr = B ? B->cr : 0;
+    if (spread) *spread = (B && B->fix) ? GLS_SPREAD_FIXED : GLS_SPREAD_FLOAT;
     return GLS_OK;
 }
 

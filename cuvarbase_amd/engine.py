@@ -137,6 +137,13 @@ class Plan:
         _lib.call("gls_plan_curve_tiles", self.handle, C.c_int32(i), C.byref(c1), C.byref(cr))
         return c1.value, cr.value
 
+    def spread(self, i):
+        """SPREAD_FLOAT or SPREAD_FIXED: how pass 1 accumulates curve i's
+        window taps (the plan's choice under SPREAD_AUTO)."""
+        v = C.c_int32(0)
+        _lib.call("gls_plan_curve_spread", self.handle, C.c_int32(i), C.byref(v))
+        return v.value
+
     def set_fork(self, enable=True):
         """Run independent size buckets concurrently on the library's streams."""
         _lib.call("gls_plan_set_fork", self.handle, C.c_int32(1 if enable else 0))
@@ -159,6 +166,13 @@ class Plan:
         _lib.call("gls_plan_stage_times", self.handle, k, p, ms, by, n)
         return [(self.STAGE_NAMES.get(k[i], str(k[i])), p[i], ms[i], by[i])
                 for i in range(n.value)]
+
+    def stage_bucket(self, i):
+        """(n1, p2, c1, cr, spread) of the size bucket stage i runs (zeros
+        for stages other than passes 1 and 2)."""
+        v = [C.c_int32(0) for _ in range(5)]
+        _lib.call("gls_plan_stage_bucket", self.handle, C.c_int32(i), *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     def destroy(self):
         h, self.handle = getattr(self, "handle", None), None
@@ -216,16 +230,19 @@ class LombBatch:
     precision : ``PRECISION_FP64_COORDS`` (FP64 time offsets, cells and
         phases; the default) or ``PRECISION_FP32`` (the reference's FP32 time
         arithmetic, for comparison)
-    spread : ``SPREAD_FLOAT`` (FP32 atomic accumulation, the default) or
-        ``SPREAD_FIXED`` (integer fixed-point accumulation: bitwise
-        reproducible results, about 18% more pass-1 time)
+    spread : ``SPREAD_AUTO`` (the default: fixed point for curves of at
+        least one point per 80 grid cells, FP32 atomics for the others),
+        ``SPREAD_FLOAT`` (FP32 atomic accumulation for every curve) or
+        ``SPREAD_FIXED`` (integer fixed-point accumulation for every curve:
+        bitwise reproducible results; slower than FP32 atomics on sparse
+        curves, faster on dense ones)
     """
 
     def __init__(self, data, freqs, sigma=2.0, m=8, mode=_lib.MODE_FLOATING_MEAN,
                  amplitude_prior=None, use_fft=True, use_double=False,
                  input_kind="raw", ybar=None, yy=None, custom_freqs=False,
                  grid_chunk_bytes=0, window=_lib.WINDOW_GAUSSIAN, window_beta=0.0,
-                 precision=_lib.PRECISION_FP64_COORDS, spread=_lib.SPREAD_FLOAT):
+                 precision=_lib.PRECISION_FP64_COORDS, spread=_lib.SPREAD_AUTO):
         if len(freqs) != len(data):
             raise ValueError("need one frequency grid per light curve")
         self.ncurves = len(data)

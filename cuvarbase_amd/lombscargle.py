@@ -261,7 +261,8 @@ class LombScargleMemory(object):
                                    out_offset=0, ybar=float(self.ybar), yy=float(self.yy))
             self._plan = Plan.lomb([curve], self.sigma, self.m, self.mode, float(self.reg[0]),
                                    float(self.reg[1]), _lib.INPUT_WEIGHTED, method,
-                                   window=win, window_beta=float(self.window_beta))
+                                   window=win, window_beta=float(self.window_beta),
+                                   spread=_lib.SPREAD_AUTO)
             self._plan_key = key
             if method == _lib.METHOD_NFFT:
                 for desc in self._descriptors():

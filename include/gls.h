@@ -119,6 +119,14 @@ typedef struct gls_params {
  * counting pass over each tile's points and a conversion of its cells: about
  * 18% more pass-1 time on the 5,000-curve survey batch. */
 #define GLS_SPREAD_FIXED 1
+/* per curve: fixed point for curves of at least one point per 80 grid cells,
+ * whose dense tiles the integer adds fill faster than FP32 atomics (pass 1
+ * 1.3-3x faster there), FP32 atomics for the sparser ones (fixed point costs
+ * 10-30% more there).  Each curve's choice depends on its own point count and
+ * grid only (gls_plan_curve_spread reports it); fixed-point curves are
+ * bitwise reproducible, the others as GLS_SPREAD_FLOAT.  No reference
+ * counterpart. */
+#define GLS_SPREAD_AUTO 2
 
 /* ---- precision of the time coordinates (gls_params.precision) ----------- */
 /* FP64 time offsets, grid cells and modulation phases, FP64 low-frequency
@@ -226,6 +234,10 @@ int gls_plan_curve_geometry(const gls_plan* plan, int32_t i, int64_t* n,
 /* Tiles the plan uses for curve i's size bucket: c1 grid columns per pass-1
  * workgroup, cr grid rows per pass-2 workgroup (zeros for the direct method). */
 int gls_plan_curve_tiles(const gls_plan* plan, int32_t i, int32_t* c1, int32_t* cr);
+/* How pass 1 accumulates curve i's window taps: GLS_SPREAD_FLOAT or
+ * GLS_SPREAD_FIXED (the plan's choice under GLS_SPREAD_AUTO; FLOAT for the
+ * direct method).  No reference counterpart. */
+int gls_plan_curve_spread(const gls_plan* plan, int32_t i, int32_t* spread);
 
 /* ---- tiles of the NFFT passes (no counterpart in the reference, whose FFTs
  * are cuFFT plans and whose gridding kernel has a fixed block size) --------
@@ -346,6 +358,12 @@ int gls_plan_enable_timing(gls_plan* plan, int32_t enable);
 int gls_plan_stage_count(const gls_plan* plan, int32_t* count);
 int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* grid,
                          float* ms, double* bytes, int32_t max);
+/* The size bucket a pass-1 or pass-2 stage runs: column length n1, log2 of
+ * the row length p2, tiles c1 and cr, and GLS_SPREAD_FLOAT or
+ * GLS_SPREAD_FIXED (its pass-1 kernel's accumulation); zeros and
+ * GLS_SPREAD_FLOAT for the other stages.  No reference counterpart. */
+int gls_plan_stage_bucket(const gls_plan* plan, int32_t stage, int32_t* n1, int32_t* p2,
+                          int32_t* c1, int32_t* cr, int32_t* spread);
 
 /* ---- adjoint NFFT (NFFTAsyncProcess.run, cuvarbase/cunfft.py:507-542) --- */
 /* y is f64 (or f32 with y_is_f32 = 1); output is complex64 (interleaved

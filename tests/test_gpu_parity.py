@@ -133,12 +133,13 @@ def test_batch_equals_alone():
     assert worst < 1e-6
 
 
-@pytest.mark.parametrize("spread", ["fixed", "float"])
+@pytest.mark.parametrize("spread", ["fixed", "float", "auto"])
 def test_survey_sample_against_oracle(spread):
     from cuvarbase_amd import _lib, engine, synthetic
 
     data, freqs = synthetic.survey(60, seed=22)
-    mode = _lib.SPREAD_FIXED if spread == "fixed" else _lib.SPREAD_FLOAT
+    mode = {"fixed": _lib.SPREAD_FIXED, "float": _lib.SPREAD_FLOAT,
+            "auto": _lib.SPREAD_AUTO}[spread]
     P = engine.LombBatch(data, freqs, sigma=2, m=8, spread=mode).run()
     worst, same = 0.0, 0
     for i in range(0, 60, 6):
@@ -150,6 +151,53 @@ def test_survey_sample_against_oracle(spread):
         assert np.all(np.isfinite(p)) and np.all((p >= 0) & (p <= 1))
     print(f"survey sample ({spread} spreading): max|dP| = {worst:.3e}, same best {same}/10")
     assert worst < TOL and same == 10
+
+
+def test_auto_spread_per_curve():
+    """GLS_SPREAD_AUTO spreads in fixed point exactly the curves of at least
+    one point per 80 grid cells: their powers are bitwise those of a
+    GLS_SPREAD_FIXED plan, the others' within FP32-atomics round-off of a
+    GLS_SPREAD_FLOAT plan, and every pass stage reports its bucket's mode."""
+    from cuvarbase_amd import _lib, engine, synthetic
+
+    data, freqs = synthetic.survey(40, seed=23)
+    rng = np.random.default_rng(23)
+    for npts, span in ((3000, 30.0), (20000, 400.0)):   # dense curves
+        t = np.sort(rng.uniform(0.0, span, npts))
+        y = np.sin(2 * np.pi * t / 1.7) + rng.normal(0.0, 0.3, npts)
+        data.append((t, y, np.full(npts, 0.3)))
+        df = 1.0 / (5.0 * span)
+        freqs.append(df + df * np.arange(int(24.0 / df)))
+    cfg = dict(sigma=1.35, m=5, window=_lib.WINDOW_ES)
+    auto = engine.LombBatch(data, freqs, spread=_lib.SPREAD_AUTO, **cfg)
+    Pa = auto.run()
+    Px = engine.LombBatch(data, freqs, spread=_lib.SPREAD_FIXED, **cfg).run()
+    Pf = engine.LombBatch(data, freqs, spread=_lib.SPREAD_FLOAT, **cfg).run()
+    nfix, worst = 0, 0.0
+    curve_modes = set()
+    for i, (t, y, dy) in enumerate(data):
+        n, n1, n2 = auto.plan.geometry(i)[:3]
+        fixed = auto.plan.spread(i) == _lib.SPREAD_FIXED
+        assert fixed == (len(t) * 80 >= n), (i, len(t), n)
+        curve_modes.add((n1, int(np.log2(n2)), auto.plan.spread(i)))
+        if fixed:
+            nfix += 1
+            assert np.array_equal(Pa[i], Px[i]), i
+        else:
+            worst = max(worst, float(np.max(np.abs(Pa[i].astype(float) - Pf[i]))))
+    print(f"auto spread: {nfix}/{len(data)} curves in fixed point; float curves vs "
+          f"GLS_SPREAD_FLOAT max|dP| = {worst:.3e}")
+    assert 2 <= nfix < len(data)
+    assert worst < 5e-5
+    stage_modes = set()
+    for k, (name, _, _, _) in enumerate(auto.plan.stage_times()):
+        n1, p2, c1, cr, sp = auto.plan.stage_bucket(k)
+        if name == "pass1":
+            stage_modes.add((n1, p2, sp))
+            assert (c1, cr) != (0, 0)
+        elif name not in ("pass2",):
+            assert (n1, p2, c1, cr) == (0, 0, 0, 0)
+    assert stage_modes == curve_modes
 
 
 def test_short_workload_against_golden():
