@@ -38,6 +38,9 @@ struct gls_plan {
         int count = 0;
         int64_t bkt0 = 0;   // bucket offsets of its first curve (the others follow)
         bool fix = false;   // fixed-point spreading in pass 1
+        // the next entry is the same size bucket in the other spreading mode:
+        // one pass-2 launch after both entries' pass 1 serves the two
+        bool pass2_with_next = false;
     };
     std::vector<Bucket> buckets;   // one entry per chunk of a size bucket and spread mode
     int spread_mode = GLS_SPREAD_FLOAT;  // gls_params.spread
@@ -320,6 +323,16 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         }
         P->buckets.push_back(B);
     }
+    // pass 2 does not depend on the spreading mode: the float and the
+    // fixed-point entries of one size bucket (adjacent in the curve list)
+    // share a launch, unless grids are chunked (a chunk's pass 2 must run
+    // before the next chunk's pass 1 reuses its region)
+    for (size_t bi = 0; bi + 1 < P->buckets.size(); ++bi) {
+        auto& B = P->buckets[bi];
+        const auto& N = P->buckets[bi + 1];
+        B.pass2_with_next = chunk <= 0 && N.n1 == B.n1 && N.p2 == B.p2 && N.cr == B.cr &&
+                            N.first == B.first + B.count;
+    }
     const int64_t grid_elems = (chunk > 0) ? grid_region : grid_total;
     P->off_nd0 = L.take(sizeof(int4) * total_pts);
     if (ngrids == 3) {
@@ -346,14 +359,20 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
     P->off_grid = L.take(sizeof(float2) * grid_elems);
     P->bcurves = bcurves;
     // lanes: buckets own disjoint grid regions unless chunked, so they may run
-    // on kAuxStreams + 1 streams at once; largest first onto the least loaded
+    // on kAuxStreams + 1 streams at once; largest first onto the least loaded.
+    // Entries that share a pass-2 launch form one group on one lane
     P->bucket_lane.assign(P->buckets.size(), 0);
     if (chunk <= 0 && P->buckets.size() > 1) {
-        std::vector<std::pair<double, int>> order;
+        std::vector<std::pair<double, int>> order;   // (-cells, first entry)
         for (size_t i = 0; i < P->buckets.size(); ++i) {
+            if (i > 0 && P->buckets[i - 1].pass2_with_next) continue;
             double cells = 0.0;
-            const auto& B = P->buckets[i];
-            for (int q = 0; q < B.count; ++q) cells += (double)P->geom[bcurves[B.first + q]].n;
+            for (size_t k = i; k < P->buckets.size(); ++k) {
+                const auto& B = P->buckets[k];
+                for (int q = 0; q < B.count; ++q)
+                    cells += (double)P->geom[bcurves[B.first + q]].n;
+                if (!B.pass2_with_next) break;
+            }
             order.push_back({-cells, (int)i});
         }
         std::sort(order.begin(), order.end());
@@ -363,7 +382,10 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
             for (int j = 1; j <= kAuxStreams; ++j)
                 if (load[j] < load[best]) best = j;
             load[best] += -o.first;
-            P->bucket_lane[o.second] = best;
+            for (size_t k = (size_t)o.second; k < P->buckets.size(); ++k) {
+                P->bucket_lane[k] = best;
+                if (!P->buckets[k].pass2_with_next) break;
+            }
         }
     }
     return GLS_OK;
@@ -486,9 +508,15 @@ void build_stage_list(gls_plan* P) {
     }
     P->stages.push_back({0, 0, prep, -1});
     if (P->pp.method == GLS_METHOD_NFFT) {
+        double b2 = 0.0;   // pass-2 bytes of the open launch
+        int b2first = 0;   // its first entry
         for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
             const auto& B = P->buckets[bi];
-            double b1 = 0.0, b2 = 0.0;
+            if (bi == 0 || !P->buckets[bi - 1].pass2_with_next) {
+                b2 = 0.0;
+                b2first = (int)bi;
+            }
+            double b1 = 0.0;
             for (int q = 0; q < B.count; ++q) {
                 const CurveGeom& G = P->geom[P->bcurves[B.first + q]];
                 const double n = (double)G.n;
@@ -496,7 +524,7 @@ void build_stage_list(gls_plan* P) {
                 b2 += ngr * 8.0 * n + (P->kind == 1 ? 8.0 : 4.0) * G.nf;
             }
             P->stages.push_back({1, B.n1 << B.p2, b1, (int)bi});
-            P->stages.push_back({2, B.n1 << B.p2, b2, (int)bi});
+            if (!B.pass2_with_next) P->stages.push_back({2, B.n1 << B.p2, b2, b2first});
         }
         if (P->kind == 0 && !P->pp.fp32_coords) {
             double bl = 0.0;
@@ -620,18 +648,24 @@ int run_plan(gls_plan* P, const double* t, const void* a, const void* b, void* o
             for (int j = 1; j <= kAuxStreams; ++j)
                 GLS_HIP_TRY(hipStreamWaitEvent(lanes[j], P->fork_join[0], 0));
         }
+        size_t gfirst = 0;   // first entry of the open pass-2 launch
         for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
             const auto& B = P->buckets[bi];
+            if (bi == 0 || !P->buckets[bi - 1].pass2_with_next) gfirst = bi;
             const hipStream_t ls = fork ? lanes[P->bucket_lane[bi]] : s;
-            const CurveGeom* bgeom = d_bgeom + B.first;
-            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, B.fix, bgeom, B.count, ngrids, B.p2, pa,
-                                     bkt + B.bkt0, grids, pp, ls));
+            GLS_HIP_TRY(launch_pass1(B.n1, B.c1, B.fix, d_bgeom + B.first, B.count, ngrids, B.p2,
+                                     pa, bkt + B.bkt0, grids, pp, ls));
             GLS_HIP_TRY(mark());
+            if (B.pass2_with_next) continue;
+            // pass 2 over this entry and the ones whose pass 2 it takes
+            const auto& F = P->buckets[gfirst];
+            const CurveGeom* g2 = d_bgeom + F.first;
+            const int n2curves = (int)(B.first + B.count - F.first);
             if (P->kind == 1)
-                GLS_HIP_TRY(launch_pass2_nfft(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
+                GLS_HIP_TRY(launch_pass2_nfft(F.p2, F.cr, F.n1, g2, state, n2curves, grids, pp,
                                               (float2*)out, ls));
             else
-                GLS_HIP_TRY(launch_pass2_lomb(B.p2, B.cr, B.n1, bgeom, state, B.count, grids, pp,
+                GLS_HIP_TRY(launch_pass2_lomb(F.p2, F.cr, F.n1, g2, state, n2curves, grids, pp,
                                               out, out_f64, ls));
             GLS_HIP_TRY(mark());
         }

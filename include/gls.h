@@ -123,9 +123,10 @@ typedef struct gls_params {
  * whose dense tiles the integer adds fill faster than FP32 atomics (pass 1
  * 1.3-3x faster there), FP32 atomics for the sparser ones (fixed point costs
  * 10-30% more there).  Each curve's choice depends on its own point count and
- * grid only (gls_plan_curve_spread reports it); fixed-point curves are
- * bitwise reproducible, the others as GLS_SPREAD_FLOAT.  No reference
- * counterpart. */
+ * grid only (gls_plan_curve_spread reports it; gls_plan_rebind keeps each
+ * curve slot's mode); fixed-point curves are bitwise reproducible, the others
+ * as GLS_SPREAD_FLOAT.  A size bucket with curves of both kinds runs two
+ * pass-1 launches and one pass-2 launch.  No reference counterpart. */
 #define GLS_SPREAD_AUTO 2
 
 /* ---- precision of the time coordinates (gls_params.precision) ----------- */

@@ -189,15 +189,30 @@ def test_auto_spread_per_curve():
           f"GLS_SPREAD_FLOAT max|dP| = {worst:.3e}")
     assert 2 <= nfix < len(data)
     assert worst < 5e-5
-    stage_modes = set()
+    # one pass-1 launch per (size bucket, mode), one pass-2 launch per size
+    # bucket (the two modes' entries share it)
+    stage_modes, pass2_sizes, npass2 = set(), set(), 0
     for k, (name, _, _, _) in enumerate(auto.plan.stage_times()):
         n1, p2, c1, cr, sp = auto.plan.stage_bucket(k)
         if name == "pass1":
             stage_modes.add((n1, p2, sp))
             assert (c1, cr) != (0, 0)
-        elif name not in ("pass2",):
+        elif name == "pass2":
+            pass2_sizes.add((n1, p2))
+            npass2 += 1
+        else:
             assert (n1, p2, c1, cr) == (0, 0, 0, 0)
     assert stage_modes == curve_modes
+    assert pass2_sizes == {(a, b) for a, b, _ in curve_modes} and npass2 == len(pass2_sizes)
+    # the concurrent launch order keeps an entry pair's pass 2 after both
+    # passes 1 (same lane): same powers
+    auto.plan.set_fork(True)
+    Pk = auto.run()
+    for i in range(len(data)):
+        if auto.plan.spread(i) == _lib.SPREAD_FIXED:
+            assert np.array_equal(Pk[i], Pa[i]), i
+        else:
+            assert float(np.max(np.abs(Pk[i].astype(float) - Pa[i]))) < 5e-5, i
 
 
 def test_short_workload_against_golden():
