@@ -600,7 +600,8 @@ def main():
             "grid_chunk_bytes": int(chunk),
             "spread": {0: "fp32 atomics", 1: "fixed point",
                        2: f"per curve: fixed point for {n_fixed} curves of at least one "
-                          "point per 80 cells, fp32 atomics for the others"}[args.spread],
+                          "point per 80 cells on columns of 64+ cells, fp32 atomics for the "
+                          "others"}[args.spread],
             "workspace_bytes": int(ws_bytes),
         },
         "roofline": {

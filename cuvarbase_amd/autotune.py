@@ -11,10 +11,12 @@ candidate tile).  For every bucket of the catalogue (1024 <= n <= 2^24) it
 plans a synthetic batch of curves with nf = n / sigma frequencies at the tuned
 NFFT parameters (sigma = 1.35, m = 5, ES window), forced onto that bucket, of
 about CELLS grid cells per grid, whose point counts follow the survey
-workload's (log-uniform over POINTS_MIN..POINTS_MAX).  It times pass 1 under
-every candidate c1 (pass 2 at its default tile), then pass 2 under every
-candidate cr (pass 1 at the chosen c1).  The times come from the plan's per-stage HIP events: the median of REPS
-executes after a warm-up.  A candidate replaces the default rule only when it
+workload's (log-uniform over POINTS_MIN..POINTS_MAX), spread as the batched
+path's default does (fixed point for the curves of at least one point per 80
+cells on columns of at least 64 cells, FP32 atomics for the others).  It times pass 1 under every candidate c1
+(pass 2 at its default tile), then pass 2 under every candidate cr (pass 1 at
+the chosen c1).  The times come from the plan's per-stage HIP events (summed
+over a bucket's launches): the median of REPS executes after a warm-up.  A candidate replaces the default rule only when it
 is at least MARGIN faster.  The bucket's cost is the time of both passes at
 its tiles per grid cell; plans then give each curve the grid of least
 cells x cost.  The table keeps every timing; build.py compiles the tiles and
@@ -127,8 +129,10 @@ class BucketBench:
         set_tiles(self.n1, self.p2, c1, cr)
         set_grid_cost(self.n1, self.p2, 1e-6)
         try:
+            # spreading as the batched path's default picks it per curve
             plan = Plan.lomb(self.curves, SIGMA, M, _lib.MODE_FLOATING_MEAN, 0.0, 0.0,
-                             _lib.INPUT_RAW, _lib.METHOD_NFFT, 0, _lib.WINDOW_ES, 0.0)
+                             _lib.INPUT_RAW, _lib.METHOD_NFFT, 0, _lib.WINDOW_ES, 0.0,
+                             spread=_lib.SPREAD_AUTO)
         finally:
             set_tiles(self.n1, self.p2, 0, 0)
             set_grid_cost(self.n1, self.p2, 0.0)
@@ -232,6 +236,8 @@ def main(argv=None):
     rows = [tune_bucket(n1, p2, buckets[(n1, p2)], args.reps) for n1, p2 in keys]
     doc = {"arch": "gfx950", "sigma": SIGMA, "m": M, "window": "exponential of semicircle",
            "cells_per_grid": CELLS, "points_per_curve": [POINTS_MIN, POINTS_MAX],
+           "spread": "per curve (fixed point at one point per 80 cells or more, "
+                     "on columns of 64 cells or more)",
            "min_cells_per_point": MIN_CELLS_PER_POINT, "reps": args.reps,
            "margin": MARGIN, "seconds": round(time.time() - t0, 1), "buckets": rows}
     with open(args.out, "w") as f:

@@ -198,8 +198,11 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        int y_is_f32, WideAux* aux, WidePart* part, int wide_max_obs,
                        hipStream_t s);
 // GLS_SPREAD_AUTO spreads a curve in fixed point when it has at least one
-// point per this many grid cells
+// point per kFixedCellsPerPoint grid cells and its pass-1 columns hold at
+// least kFixedMinColumn cells (on shorter columns the per-row counting and
+// conversion cost more than the integer adds save)
 constexpr int kFixedCellsPerPoint = 80;
+constexpr int kFixedMinColumn = 64;
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
 // (c1: the bucket's pass-1 tile, cr: its pass-2 tile; fix: fixed-point

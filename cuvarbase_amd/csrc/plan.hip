@@ -279,9 +279,10 @@ int plan_nfft_geometry(gls_plan* P, int ngrids, Layout& L, int64_t total_pts,
         G.ninv = 1.0f / (float)n;
         // GLS_SPREAD_AUTO: fixed point for curves of at least one point per
         // kFixedCellsPerPoint grid cells, where its integer adds outrun the
-        // FP32 atomics of dense tiles
+        // FP32 atomics of dense tiles, on columns of kFixedMinColumn cells or
+        // more
         const int fix = P->spread_mode == GLS_SPREAD_FIXED ||
-                        (P->spread_mode == GLS_SPREAD_AUTO &&
+                        (P->spread_mode == GLS_SPREAD_AUTO && n1 >= kFixedMinColumn &&
                          (int64_t)G.n_obs * kFixedCellsPerPoint >= n);
         P->curve_fixed[i] = (int8_t)fix;
         by_size[{n1, p2, fix}].push_back(i);

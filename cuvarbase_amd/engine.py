@@ -231,7 +231,8 @@ class LombBatch:
         phases; the default) or ``PRECISION_FP32`` (the reference's FP32 time
         arithmetic, for comparison)
     spread : ``SPREAD_AUTO`` (the default: fixed point for curves of at
-        least one point per 80 grid cells, FP32 atomics for the others),
+        least one point per 80 grid cells on pass-1 columns of at least 64
+        cells, FP32 atomics for the others),
         ``SPREAD_FLOAT`` (FP32 atomic accumulation for every curve) or
         ``SPREAD_FIXED`` (integer fixed-point accumulation for every curve:
         bitwise reproducible results; slower than FP32 atomics on sparse

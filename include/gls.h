@@ -119,10 +119,11 @@ typedef struct gls_params {
  * counting pass over each tile's points and a conversion of its cells: about
  * 18% more pass-1 time on the 5,000-curve survey batch. */
 #define GLS_SPREAD_FIXED 1
-/* per curve: fixed point for curves of at least one point per 80 grid cells,
- * whose dense tiles the integer adds fill faster than FP32 atomics (pass 1
- * 1.3-3x faster there), FP32 atomics for the sparser ones (fixed point costs
- * 10-30% more there).  Each curve's choice depends on its own point count and
+/* per curve: fixed point for curves of at least one point per 80 grid cells
+ * on pass-1 columns of 64 cells or more, whose dense tiles the integer adds
+ * fill faster than FP32 atomics (pass 1 1.3-3x faster there), FP32 atomics
+ * for the others (fixed point costs 10-30% more on sparse tiles, and more on
+ * short columns, where the per-row counting and conversion weigh most).  Each curve's choice depends on its own point count and
  * grid only (gls_plan_curve_spread reports it; gls_plan_rebind keeps each
  * curve slot's mode); fixed-point curves are bitwise reproducible, the others
  * as GLS_SPREAD_FLOAT.  A size bucket with curves of both kinds runs two
