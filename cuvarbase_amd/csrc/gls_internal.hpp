@@ -197,6 +197,12 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
                        PointArrays pa, int32_t* bkt, const PlanParams& pp,
                        int y_is_f32, WideAux* aux, WidePart* part, int wide_max_obs,
                        hipStream_t s);
+// work-items of one launch of the NFFT passes: the dispatch counts a grid's
+// work-items in 32 bits, so a size bucket with more goes out in several
+// launches over consecutive curves (max_launch_items(): this, or a smaller
+// cap set by gls_launch_items_set)
+constexpr int64_t kMaxLaunchItems = (int64_t)1 << 31;
+int64_t max_launch_items();
 // GLS_SPREAD_AUTO spreads a curve in fixed point when it has at least one
 // point per kFixedCellsPerPoint grid cells and its pass-1 columns hold at
 // least kFixedMinColumn cells (on shorter columns the per-row counting and

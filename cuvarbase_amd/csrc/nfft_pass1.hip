@@ -11,6 +11,7 @@
 // fast_gaussian_grid, cuvarbase/kernels/cunfft.cu:115-163) from the points the
 // prep kernel bucketed by column block, so the spread grid never touches HBM.
 // A is stored column-major (A[u2*n1 + k1]) so pass 1 writes whole columns.
+#include <algorithm>
 #include <type_traits>
 
 #include "device_util.hpp"
@@ -252,21 +253,28 @@ template <int N1, int C1>
 hipError_t pass1_dispatch(bool fix, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
                           PointArrays pa, const int32_t* bkt, float2* grids,
                           const PlanParams& pp, hipStream_t s) {
+    constexpr int NT = pass1_threads(N1, C1);
     const int nb = (1 << p2) / C1;
-    const int64_t nblocks = (int64_t)ncurves * ngrids * nb;
-    if (nblocks > 0x7fffffff || nb < 1) return hipErrorInvalidValue;
+    if (nb < 1 || ngrids < 1) return hipErrorInvalidValue;
     const float ainv = 1.0f / ((float)pp.m + 0.5f);
-    if (fix)
-        hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), dim3((unsigned)nblocks),
-                           dim3(pass1_threads(N1, C1)), 0,
-                           s, bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
-                           ainv);
-    else
-        hipLaunchKernelGGL((pass1_kernel<N1, C1, false>), dim3((unsigned)nblocks),
-                           dim3(pass1_threads(N1, C1)), 0,
-                           s, bgeom, ngrids, p2, pa, bkt, grids, pp.m, pp.window, pp.es_beta,
-                           ainv);
-    return hipGetLastError();
+    // curves per launch: blocks x threads within max_launch_items()
+    const int64_t per_curve = (int64_t)ngrids * nb * NT;
+    const int chunk = (int)std::min<int64_t>(ncurves, std::max<int64_t>(1, max_launch_items() / per_curve));
+    for (int c0 = 0; c0 < ncurves; c0 += chunk) {
+        const int cnt = std::min(chunk, ncurves - c0);
+        const dim3 grid((unsigned)((int64_t)cnt * ngrids * nb));
+        // a curve's bucket offsets follow its predecessors', 2 (nb + 1) each
+        const int32_t* bk = bkt + (int64_t)c0 * 2 * (nb + 1);
+        if (fix)
+            hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), grid, dim3(NT), 0, s, bgeom + c0,
+                               ngrids, p2, pa, bk, grids, pp.m, pp.window, pp.es_beta, ainv);
+        else
+            hipLaunchKernelGGL((pass1_kernel<N1, C1, false>), grid, dim3(NT), 0, s, bgeom + c0,
+                               ngrids, p2, pa, bk, grids, pp.m, pp.window, pp.es_beta, ainv);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace

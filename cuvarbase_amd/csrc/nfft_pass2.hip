@@ -9,6 +9,7 @@
 // (normalize, cuvarbase/kernels/cunfft.cu:243-245) and forms the GLS power
 // (lspow, cuvarbase/kernels/lomb.cu:126-141, in closed form) without ever
 // writing the transforms.
+#include <algorithm>
 #include <type_traits>
 
 #include "device_util.hpp"
@@ -166,12 +167,19 @@ __global__ __launch_bounds__(pass2_threads_for(P2, CR, OUT)) void pass2_kernel(
 template <int P2, int CR, int OUT>
 hipError_t pass2_dispatch(int n1, const CurveGeom* bgeom, const CurveState* state, int ncurves,
                           const float2* grids, const PlanParams& pp, void* out, hipStream_t s) {
-    const int64_t nblocks = (int64_t)ncurves * ((n1 + CR - 1) / CR);
-    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pass2_kernel<P2, CR, OUT>), dim3((unsigned)nblocks),
-                       dim3(pass2_threads_for(P2, CR, OUT)), 0, s, bgeom, state, n1, grids, pp,
-                       out);
-    return hipGetLastError();
+    constexpr int NT = pass2_threads_for(P2, CR, OUT);
+    const int64_t per_curve = (int64_t)((n1 + CR - 1) / CR) * NT;
+    // curves per launch: blocks x threads within max_launch_items()
+    const int chunk = (int)std::min<int64_t>(ncurves, std::max<int64_t>(1, max_launch_items() / per_curve));
+    for (int c0 = 0; c0 < ncurves; c0 += chunk) {
+        const int cnt = std::min(chunk, ncurves - c0);
+        hipLaunchKernelGGL((pass2_kernel<P2, CR, OUT>),
+                           dim3((unsigned)((int64_t)cnt * ((n1 + CR - 1) / CR))), dim3(NT), 0, s,
+                           bgeom + c0, state, n1, grids, pp, out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // the (p2, cr) pairs this library has kernels for

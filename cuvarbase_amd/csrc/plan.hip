@@ -119,6 +119,9 @@ double gauss_b(double sigma, int m) {
     return 2.0 * sigma * (double)m / ((2.0 * sigma - 1.0) * M_PI);
 }
 
+// work-item cap of one NFFT-pass launch (gls_launch_items_set)
+std::atomic<int64_t> g_max_launch_items{kMaxLaunchItems};
+
 // ---- tiles and measured costs of the size buckets ------------------------------
 std::mutex g_tile_mu;
 std::map<std::pair<int, int>, BucketTiles> g_tile_override;   // gls_tile_set
@@ -472,6 +475,8 @@ const std::vector<float>& es_deconv_table(int m, float beta) {
 }
 
 }  // namespace
+
+int64_t max_launch_items() { return g_max_launch_items.load(std::memory_order_relaxed); }
 
 int es_table_device(int m, float beta, const float** out) {
     static std::mutex mu;
@@ -1003,6 +1008,13 @@ int gls_tile_set(int32_t n1, int32_t p2, int32_t c1, int32_t cr) {
     std::lock_guard<std::mutex> lk(g_tile_mu);
     if (!c1 && !cr) g_tile_override.erase({n1, p2});
     else g_tile_override[{n1, p2}] = BucketTiles{c1, cr};
+    return GLS_OK;
+}
+
+int gls_launch_items_set(int64_t items) {
+    if (items > kMaxLaunchItems)
+        return fail(GLS_E_INVALID, "at most 2^31 work-items per launch");
+    g_max_launch_items.store(items > 0 ? items : kMaxLaunchItems);
     return GLS_OK;
 }
 

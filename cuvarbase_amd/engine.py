@@ -55,6 +55,13 @@ def get_grid_cost(n1, p2):
     return c.value, src.value
 
 
+def set_launch_items(items):
+    """Cap the work-items of one NFFT-pass launch (at most and by default
+    2^31; a size bucket with more runs as several launches over consecutive
+    curves); ``items <= 0`` restores the default."""
+    _lib.call("gls_launch_items_set", C.c_int64(int(items)))
+
+
 class Plan:
     """Owns a ``gls_plan`` handle."""
 

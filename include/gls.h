@@ -254,6 +254,13 @@ int gls_plan_curve_spread(const gls_plan* plan, int32_t i, int32_t* spread);
  * gls_tile_get returns the tiles a plan made now would use. */
 int gls_tile_set(int32_t n1, int32_t p2, int32_t c1, int32_t cr);
 int gls_tile_get(int32_t n1, int32_t p2, int32_t* c1, int32_t* cr);
+
+/* Work-items of one launch of an NFFT pass: at most 2^31, the default (a
+ * dispatch counts its work-items in 32 bits).  A size bucket with more runs as
+ * several launches over consecutive curves; a smaller cap (set before the
+ * execute, for every plan) splits launches further, as tests of that path
+ * do.  items <= 0 restores the default.  No reference counterpart. */
+int gls_launch_items_set(int64_t items);
 /* A plan gives each curve the grid n = n1 * 2^p2 >= sigma * nf of least
  * n x (cost per cell of its bucket): the time of the two passes per cell
  * measured on the device (the same table), or a model (one pass over the

@@ -216,6 +216,28 @@ def test_auto_spread_per_curve():
             assert float(np.max(np.abs(Pk[i].astype(float) - Pa[i]))) < 5e-5, i
 
 
+def test_pass_launches_split_over_curves():
+    """A size bucket whose pass-1 or pass-2 grid holds more work-items than
+    one launch may (2^31: a dispatch counts them in 32 bits) runs as several
+    launches over consecutive curves.  With the cap lowered so that a small
+    batch splits every bucket into launches of a few curves, fixed-point
+    spreading gives bitwise the powers of the unsplit launches."""
+    from cuvarbase_amd import _lib, engine, synthetic
+
+    data, freqs = synthetic.survey(60, seed=31)
+    cfg = dict(sigma=1.35, m=5, window=_lib.WINDOW_ES, spread=_lib.SPREAD_FIXED)
+    ref = engine.LombBatch(data, freqs, **cfg).run()
+    try:
+        engine.set_launch_items(300_000)
+        got = engine.LombBatch(data, freqs, **cfg).run()
+    finally:
+        engine.set_launch_items(0)
+    for i, (a, b) in enumerate(zip(ref, got)):
+        assert np.array_equal(a, b), i
+    with pytest.raises(_lib.GLSError):
+        engine.set_launch_items((1 << 31) + 1)
+
+
 def test_short_workload_against_golden():
     from cuvarbase_amd import synthetic
 
