@@ -203,6 +203,9 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
 // cap set by gls_launch_items_set)
 constexpr int64_t kMaxLaunchItems = (int64_t)1 << 31;
 int64_t max_launch_items();
+// curves of one plan: the per-curve launches (preparation, low-frequency
+// sums, top-k: at most 512 threads per curve) stay within kMaxLaunchItems
+constexpr int kMaxPlanCurves = 1 << 22;
 // GLS_SPREAD_AUTO spreads a curve in fixed point when it has at least one
 // point per kFixedCellsPerPoint grid cells and its pass-1 columns hold at
 // least kFixedMinColumn cells (on shorter columns the per-row counting and

@@ -411,7 +411,9 @@ int plan_direct_tiles(gls_plan* P, std::vector<int32_t>& tiles) {
             tf.push_back(k);
         }
     }
-    if (tc.size() > 0x3fffffff) return fail(GLS_E_UNSUPPORTED, "too many direct-sum tiles");
+    // one launch of 256-thread workgroups, one per tile
+    if ((int64_t)tc.size() * 256 > kMaxLaunchItems)
+        return fail(GLS_E_UNSUPPORTED, "too many direct-sum tiles for one launch: split the batch");
     P->rf = rf;
     P->ntiles = (int)tc.size();
     tiles = tc;
@@ -740,6 +742,8 @@ int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves, int32_t ncurv
     if (!plan || !params) return fail(GLS_E_INVALID, "plan or params is NULL");
     *plan = nullptr;
     if (ncurves < 0 || (ncurves > 0 && !curves)) return fail(GLS_E_INVALID, "bad curve list");
+    if (ncurves > kMaxPlanCurves)
+        return fail(GLS_E_UNSUPPORTED, "more than 2^22 curves in one plan: split the batch");
     const gls_params& prm = *params;
     if (prm.mode < 0 || prm.mode > 2) return fail(GLS_E_INVALID, "mode must be 0, 1 or 2");
     if (prm.input_kind != GLS_INPUT_RAW && prm.input_kind != GLS_INPUT_WEIGHTED)
@@ -869,6 +873,8 @@ int gls_nfft_plan_create(gls_plan** plan, const gls_nfft_item* items, int32_t ni
     if (!plan) return fail(GLS_E_INVALID, "plan is NULL");
     *plan = nullptr;
     if (nitems < 0 || (nitems > 0 && !items)) return fail(GLS_E_INVALID, "bad item list");
+    if (nitems > kMaxPlanCurves)
+        return fail(GLS_E_UNSUPPORTED, "more than 2^22 items in one plan: split the batch");
     if (!(sigma >= 1.0 && sigma <= 64.0)) return fail(GLS_E_INVALID, "sigma must be in [1, 64]");
     if (m < 1 || m > 24) return fail(GLS_E_INVALID, "m must be in [1, 24]");
     gls_plan* P = new gls_plan();

@@ -216,7 +216,9 @@ int gls_mem_info(size_t* free_bytes, size_t* total_bytes);
  * and lays out the workspace.  Replaces LombScargleMemory.allocate +
  * NFFTMemory.allocate_grid + the two cuFFT plans per curve
  * (cuvarbase/lombscargle.py:129-181, cuvarbase/cunfft.py:82-92).
- * `curves` is read during the call only. */
+ * `curves` is read during the call only.  At most 2^22 curves per plan
+ * (GLS_E_UNSUPPORTED above: split the batch), so that no per-curve launch
+ * exceeds 2^31 work-items. */
 int gls_lomb_plan_create(gls_plan** plan, const gls_curve* curves,
                          int32_t ncurves, const gls_params* params);
 /* Plans are host objects (no device memory): their descriptors are copied
