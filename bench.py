@@ -377,10 +377,10 @@ def main():
     # time over the instrumented steps
     by_kernel, by_kind = {}, {}
     for (i, name), (c, ms, by) in stage_acc.items():
-        n1, p2, c1, cr, sp = batch.plan.stage_bucket(i)
+        n1, p2, c1, cr, sp, nts = batch.plan.stage_bucket(i)
         if name == "pass1":
             fx = "true" if sp == _lib.SPREAD_FIXED else "false"
-            key = f"pass1_kernel<{n1}, {c1}, {fx}>"
+            key = f"pass1_kernel<{n1}, {c1}, {fx}, {'true' if nts else 'false'}>"
         elif name == "pass2":
             key = f"pass2_kernel<{p2}, {cr}, 0>"
         else:

@@ -125,7 +125,7 @@ _PROTOS = {
     "gls_plan_curve_tiles": [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "gls_plan_curve_spread": [_vp, C.c_int32, C.POINTER(C.c_int32)],
     "gls_launch_items_set": [C.c_int64],
-    "gls_plan_stage_bucket": [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 5,
+    "gls_plan_stage_bucket": [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 6,
     "gls_tile_set": [C.c_int32, C.c_int32, C.c_int32, C.c_int32],
     "gls_tile_get": [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "gls_grid_cost_set": [C.c_int32, C.c_int32, C.c_double],

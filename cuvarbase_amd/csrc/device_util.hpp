@@ -127,10 +127,14 @@ __device__ __forceinline__ float2 grid_load(__amdgpu_buffer_rsrc_t r, uint32_t v
         __builtin_bit_cast(gls_v2f, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
     return make_float2(v.x, v.y);
 }
+// CPOL: cache policy of the store, 0 the default, kStoreNonTemporal streaming
+// (the line is not kept in the caches)
+constexpr int kStoreNonTemporal = 2;
+template <int CPOL = 0>
 __device__ __forceinline__ void grid_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
                                            float2 v) {
     const gls_v2f x = {v.x, v.y};
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(gls_v2u, x), r, voff, soff, CPOL);
 }
 __device__ __forceinline__ void buf_store_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, 0u, 0);

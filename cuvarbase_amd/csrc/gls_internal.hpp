@@ -203,6 +203,9 @@ hipError_t launch_prep(const CurveGeom* d_geom, const int32_t* small, int nsmall
 // cap set by gls_launch_items_set)
 constexpr int64_t kMaxLaunchItems = (int64_t)1 << 31;
 int64_t max_launch_items();
+// pass 1 stores a size bucket's grids non-temporally when they exceed this
+// (the memory-side cache holds 256 MB)
+constexpr int64_t kStreamStoreBytes = (int64_t)96 << 20;
 // curves of one plan: the per-curve launches (preparation, low-frequency
 // sums, top-k: at most 512 threads per curve) stay within kMaxLaunchItems
 constexpr int kMaxPlanCurves = 1 << 22;
@@ -215,9 +218,9 @@ constexpr int kFixedMinColumn = 64;
 // pass 1 and 2 of one size bucket: bgeom holds the geometry of its ncurves
 // curves, one after another (a block reads its curve's entry directly)
 // (c1: the bucket's pass-1 tile, cr: its pass-2 tile; fix: fixed-point
-// spreading); pass 1's bkt points at the first curve's bucket offsets, which
-// the others' follow, 2 (nb + 1) each
-hipError_t launch_pass1(int n1, int c1, bool fix, const CurveGeom* bgeom, int ncurves,
+// spreading; nts: non-temporal grid stores); pass 1's bkt points at the first
+// curve's bucket offsets, which the others' follow, 2 (nb + 1) each
+hipError_t launch_pass1(int n1, int c1, bool fix, bool nts, const CurveGeom* bgeom, int ncurves,
                         int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s);
 hipError_t launch_pass2_lomb(int p2, int cr, int n1, const CurveGeom* bgeom,

@@ -39,7 +39,9 @@ constexpr int pass1_threads(int n1, int c1) { return n1 * c1 >= 6144 ? 512 : 256
 // row can exceed 2^30.  Counts and maxima are integer LDS atomics, so e, the
 // roundings and the sums do not depend on the order of the points.
 // C1: grid columns per workgroup (one LDS tile of N1 x C1 cells)
-template <int N1, int C1, bool FIX>
+// NTS: the grids are stored non-temporally (launches whose grids are larger
+// than the memory-side cache; pass 2 reads them from HBM either way)
+template <int N1, int C1, bool FIX, bool NTS>
 __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const CurveGeom* __restrict__ bgeom,
                                                     int ngrids, int p2, PointArrays pa,
                                                     const int32_t* __restrict__ bkt,
@@ -238,19 +240,21 @@ __global__ __launch_bounds__(pass1_threads(N1, C1)) void pass1_kernel(const Curv
         // block height, a butterfly's outputs share their in-block offset and
         // step whole row blocks (the step goes to the scalar offset)
         constexpr int R = kRowBlock;
+        constexpr int CP = NTS ? kStoreNonTemporal : 0;
         if ((ns % R) == 0) {
-            grid_store(out, 8u * (uint32_t)((kb / R) * rb + (a + jj) * R + kb % R),
-                       8u * (uint32_t)(q * (ns / R) * rb), cmul(v, w));
+            grid_store<CP>(out, 8u * (uint32_t)((kb / R) * rb + (a + jj) * R + kb % R),
+                           8u * (uint32_t)(q * (ns / R) * rb), cmul(v, w));
         } else {
             const int k1 = kb + q * ns;
-            grid_store(out, 8u * (uint32_t)((k1 / R) * rb + (a + jj) * R + k1 % R), 0u,
-                       cmul(v, w));
+            grid_store<CP>(out, 8u * (uint32_t)((k1 / R) * rb + (a + jj) * R + k1 % R), 0u,
+                           cmul(v, w));
         }
     });
 }
 
 template <int N1, int C1>
-hipError_t pass1_dispatch(bool fix, const CurveGeom* bgeom, int ncurves, int ngrids, int p2,
+hipError_t pass1_dispatch(bool fix, bool nts, const CurveGeom* bgeom, int ncurves, int ngrids,
+                          int p2,
                           PointArrays pa, const int32_t* bkt, float2* grids,
                           const PlanParams& pp, hipStream_t s) {
     constexpr int NT = pass1_threads(N1, C1);
@@ -265,12 +269,17 @@ hipError_t pass1_dispatch(bool fix, const CurveGeom* bgeom, int ncurves, int ngr
         const dim3 grid((unsigned)((int64_t)cnt * ngrids * nb));
         // a curve's bucket offsets follow its predecessors', 2 (nb + 1) each
         const int32_t* bk = bkt + (int64_t)c0 * 2 * (nb + 1);
-        if (fix)
-            hipLaunchKernelGGL((pass1_kernel<N1, C1, true>), grid, dim3(NT), 0, s, bgeom + c0,
-                               ngrids, p2, pa, bk, grids, pp.m, pp.window, pp.es_beta, ainv);
-        else
-            hipLaunchKernelGGL((pass1_kernel<N1, C1, false>), grid, dim3(NT), 0, s, bgeom + c0,
-                               ngrids, p2, pa, bk, grids, pp.m, pp.window, pp.es_beta, ainv);
+#define GLS_P1_LAUNCH(F, S)                                                                    \
+    hipLaunchKernelGGL((pass1_kernel<N1, C1, F, S>), grid, dim3(NT), 0, s, bgeom + c0, ngrids, \
+                       p2, pa, bk, grids, pp.m, pp.window, pp.es_beta, ainv)
+        if (fix) {
+            if (nts) GLS_P1_LAUNCH(true, true);
+            else GLS_P1_LAUNCH(true, false);
+        } else {
+            if (nts) GLS_P1_LAUNCH(false, true);
+            else GLS_P1_LAUNCH(false, false);
+        }
+#undef GLS_P1_LAUNCH
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -288,13 +297,13 @@ bool pass1_tile_built(int n1, int c1) {
 }
 
 // the (n1, c1) pairs this library has kernels for
-hipError_t launch_pass1(int n1, int c1, bool fix, const CurveGeom* bgeom, int ncurves,
+hipError_t launch_pass1(int n1, int c1, bool fix, bool nts, const CurveGeom* bgeom, int ncurves,
                         int ngrids, int p2, PointArrays pa, const int32_t* bkt, float2* grids,
                         const PlanParams& pp, hipStream_t s) {
     if (ncurves <= 0) return hipSuccess;
 #define GLS_P1_CASE(N, C)  \
     if (n1 == N && c1 == C) \
-        return pass1_dispatch<N, C>(fix, bgeom, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
+        return pass1_dispatch<N, C>(fix, nts, bgeom, ncurves, ngrids, p2, pa, bkt, grids, pp, s);
     GLS_PASS1_TILES(GLS_P1_CASE)
 #undef GLS_P1_CASE
     return hipErrorInvalidValue;

@@ -175,9 +175,10 @@ class Plan:
                 for i in range(n.value)]
 
     def stage_bucket(self, i):
-        """(n1, p2, c1, cr, spread) of the size bucket stage i runs (zeros
-        for stages other than passes 1 and 2)."""
-        v = [C.c_int32(0) for _ in range(5)]
+        """(n1, p2, c1, cr, spread, streamed) of the size bucket stage i runs
+        (zeros for stages other than passes 1 and 2; streamed: pass 1 stores
+        the grids non-temporally)."""
+        v = [C.c_int32(0) for _ in range(6)]
         _lib.call("gls_plan_stage_bucket", self.handle, C.c_int32(i), *[C.byref(x) for x in v])
         return tuple(x.value for x in v)
 

@@ -370,11 +370,13 @@ int gls_plan_stage_count(const gls_plan* plan, int32_t* count);
 int gls_plan_stage_times(const gls_plan* plan, int32_t* kind, int32_t* grid,
                          float* ms, double* bytes, int32_t max);
 /* The size bucket a pass-1 or pass-2 stage runs: column length n1, log2 of
- * the row length p2, tiles c1 and cr, and GLS_SPREAD_FLOAT or
- * GLS_SPREAD_FIXED (its pass-1 kernel's accumulation); zeros and
- * GLS_SPREAD_FLOAT for the other stages.  No reference counterpart. */
+ * the row length p2, tiles c1 and cr, GLS_SPREAD_FLOAT or GLS_SPREAD_FIXED
+ * (its pass-1 kernel's accumulation) and streamed = 1 when pass 1 stores the
+ * bucket's grids non-temporally (grids of more than 96 MB, which the 256 MB
+ * memory-side cache would not keep until pass 2); zeros and GLS_SPREAD_FLOAT
+ * for the other stages.  No reference counterpart. */
 int gls_plan_stage_bucket(const gls_plan* plan, int32_t stage, int32_t* n1, int32_t* p2,
-                          int32_t* c1, int32_t* cr, int32_t* spread);
+                          int32_t* c1, int32_t* cr, int32_t* spread, int32_t* streamed);
 
 /* ---- adjoint NFFT (NFFTAsyncProcess.run, cuvarbase/cunfft.py:507-542) --- */
 /* y is f64 (or f32 with y_is_f32 = 1); output is complex64 (interleaved

@@ -194,7 +194,7 @@ def test_auto_spread_per_curve():
     # bucket (the two modes' entries share it)
     stage_modes, pass2_sizes, npass2 = set(), set(), 0
     for k, (name, _, _, _) in enumerate(auto.plan.stage_times()):
-        n1, p2, c1, cr, sp = auto.plan.stage_bucket(k)
+        n1, p2, c1, cr, sp, _ = auto.plan.stage_bucket(k)
         if name == "pass1":
             stage_modes.add((n1, p2, sp))
             assert (c1, cr) != (0, 0)
@@ -236,6 +236,29 @@ def test_pass_launches_split_over_curves():
         assert np.array_equal(a, b), i
     with pytest.raises(_lib.GLSError):
         engine.set_launch_items((1 << 31) + 1)
+
+
+def test_streamed_grid_stores():
+    """Size buckets whose grids exceed 96 MB are stored by pass 1
+    non-temporally; with fixed-point spreading the powers are bitwise those of
+    the same curves in small batches, whose buckets are stored normally."""
+    from cuvarbase_amd import _lib, engine, synthetic
+
+    data, freqs = synthetic.survey(400, seed=33)
+    cfg = dict(sigma=1.35, m=5, window=_lib.WINDOW_ES, spread=_lib.SPREAD_FIXED)
+    big = engine.LombBatch(data, freqs, **cfg)
+    P = big.run()
+    streamed = [big.plan.stage_bucket(k)[5] for k, st in enumerate(big.plan.stage_times())
+                if st[0] == "pass1"]
+    assert any(streamed) and not all(streamed), streamed
+    # 5 curves of at most 3 x 786,432 cells each: under 96 MB in any bucket
+    for i0 in range(0, 400, 5):
+        small = engine.LombBatch(data[i0:i0 + 5], freqs[i0:i0 + 5], **cfg)
+        Q = small.run()
+        assert not any(small.plan.stage_bucket(k)[5]
+                       for k, st in enumerate(small.plan.stage_times()) if st[0] == "pass1")
+        for j, q in enumerate(Q):
+            assert np.array_equal(q, P[i0 + j]), i0 + j
 
 
 def test_short_workload_against_golden():
