@@ -10,7 +10,9 @@
 // Pass 1 builds its columns directly in LDS by window spreading (the work of
 // fast_gaussian_grid, cuvarbase/kernels/cunfft.cu:115-163) from the points the
 // prep kernel bucketed by column block, so the spread grid never touches HBM.
-// A is stored column-major (A[u2*n1 + k1]) so pass 1 writes whole columns.
+// A is stored in row blocks of R = kRowBlock rows, A[(k1/R) R n2 + u2 R + k1 % R]
+// (gls_internal.hpp), so a column goes out as segments of R cells and a
+// pass-2 tile of R rows is one contiguous run.
 #include <algorithm>
 #include <type_traits>
 
@@ -32,12 +34,12 @@ constexpr int pass1_threads(int n1, int c1) { return n1 * c1 >= 6144 ? 512 : 256
 // FIX: fixed-point spreading (GLS_SPREAD_FIXED): the tile holds int32 pairs
 // while the taps are added, each tap rounded to a multiple of 2^-e and added
 // as one 64-bit integer im 2^32 + re, then is converted to float2 in place
-// before the FFT.  All the cells a point touches
-// in a tile lie in one row (its columns are consecutive and do not wrap), so
-// e is chosen per row: from the number of the tile's points in the row times
-// their largest |component| (window weights are at most 1), no cell of the
-// row can exceed 2^30.  Counts and maxima are integer LDS atomics, so e, the
-// roundings and the sums do not depend on the order of the points.
+// before the FFT.  All the cells a point touches in a tile lie in one row (its
+// columns are consecutive and do not wrap), so e is chosen per row: from the
+// number of the tile's points in the row times their largest |component|
+// (window weights are at most 1), no cell of the row can exceed 2^30.  Counts
+// and maxima are integer LDS atomics, so e, the roundings and the sums do not
+// depend on the order of the points.
 // C1: grid columns per workgroup (one LDS tile of N1 x C1 cells)
 // NTS: the grids are stored non-temporally (launches whose grids are larger
 // than the memory-side cache; pass 2 reads them from HBM either way)

@@ -1,9 +1,10 @@
 // Pass 2 of the batched NFFT-GLS pipeline: row FFTs of the four-step split,
 // the window deconvolution and the GLS power assembly (or the NFFT output).
 //
-// Grid of n = n1*n2 cells (n2 = 2^p2); pass 1 has left A[k1,u2] column-major
-// (A[u2*n1 + k1]) with the inter-pass twiddle applied, so row k1 of the
-// transform is G[k1 + n1 k2] = sum_u2 A[k1,u2] exp(2 pi i u2 k2/n2).  A
+// Grid of n = n1*n2 cells (n2 = 2^p2); pass 1 has left A[k1,u2] in row blocks
+// of R = kRowBlock rows (A[(k1/R) R n2 + u2 R + k1 % R]) with the inter-pass
+// twiddle applied, so row k1 of the transform is
+// G[k1 + n1 k2] = sum_u2 A[k1,u2] exp(2 pi i u2 k2/n2).  A
 // workgroup transforms CR rows of the three grids of a curve (SYW, SW(f),
 // SW(2f) -- all indexed by the same output k), applies the deconvolution
 // (normalize, cuvarbase/kernels/cunfft.cu:243-245) and forms the GLS power
